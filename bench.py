@@ -1,0 +1,204 @@
+"""Benchmark: plain FP64 rocblas_dtrsm_strided_batched on one MI355X.
+
+Workload (one "step"): 20000 independent triangular solves of m=32, n=16 for
+each of the eight side/uplo/trans cases {L,R} x {L,U} x {N,T}, diag=N, alpha=1,
+lda = k, ldb = m, strides lda*k and ldb*n.  A = (diagonal U[1,2], off-diagonal
+U[-1,1]/k) / k, B = U[-1,1], SplitMix64 seed 0 (rocblas_amd.synth).  Inputs are
+resident in HBM before timing; each step solves in place (eight launches).
+
+Printed JSON (one line): value = solves per second over the timed steps;
+roofline = the side-left kernel's algorithmic HBM bytes (8*(k(k+1)/2 + 2mn) per
+instance) over its average launch duration measured with HIP events on the
+launch stream; cpu_baseline = the oracle's netlib loop nest, OpenMP over
+instances, on this host's cores (a bounded sample of the same workload).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+CASES = [(s, u, t) for s in "LR" for u in "LU" for t in "NT"]
+BATCH, M, N = 20000, 32, 16
+
+
+def algo_bytes(side):
+    k = M if side == "L" else N
+    return 8 * (k * (k + 1) // 2 + 2 * M * N)
+
+
+def algo_flops(side):
+    k = M if side == "L" else N
+    return M * N * k
+
+
+def make_inputs(torch, synth):
+    data = {}
+    Bsrc = {}
+    for side in "LR":
+        Bm = synth.rhs_uniform(M, N, 0, batch=BATCH)
+        Bsrc[side] = np.ascontiguousarray(np.swapaxes(Bm, 1, 2)).ravel()
+    for side, uplo, trans in CASES:
+        k = M if side == "L" else N
+        A = synth.tri_uniform(k, uplo, 0, batch=BATCH, scale=1.0 / k)
+        Af = np.ascontiguousarray(np.swapaxes(A, 1, 2)).ravel()
+        data[(side, uplo, trans)] = (Af, torch.from_numpy(Af).cuda(),
+                                     torch.from_numpy(Bsrc[side].copy()).cuda())
+    return data, Bsrc
+
+
+def cpu_baseline(oracle, data, Bsrc, budget_s=12.0):
+    """Oracle netlib loop nest, OpenMP over instances, all eight cases; repeats the
+    full workload until ~budget_s of CPU wall time (at least once)."""
+    threads = oracle.num_threads()
+    reps, t_total, solves = 0, 0.0, 0
+    while reps == 0 or t_total < budget_s:
+        for (side, uplo, trans), (Af, _, _) in data.items():
+            k = M if side == "L" else N
+            Bw = Bsrc[side].copy()
+            t0 = time.perf_counter()
+            oracle.trsm_batched_omp(side, uplo, trans, "N", M, N, 1.0, Af, k, k * k, Bw, M, M * N,
+                                    BATCH, threads)
+            t_total += time.perf_counter() - t0
+            solves += BATCH
+        reps += 1
+        if reps >= 50:
+            break
+    return {"value": solves / t_total, "unit": "solves/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{reps} pass(es) over all 8 cases x {BATCH} instances (m={M}, n={N}), "
+                      f"{t_total:.1f} s"}
+
+
+def check_accuracy(torch, rb, h, data, Bsrc):
+    """Fresh solve of every case; per-instance normwise backward error in float64."""
+    worst = 0.0
+    for (side, uplo, trans), (Af, dA, _) in data.items():
+        k = M if side == "L" else N
+        dB = torch.from_numpy(Bsrc[side].copy()).cuda()
+        rb.check(rb.trsm_strided_batched(h, side, uplo, trans, "N", M, N, 1.0, dA.data_ptr(), k,
+                                         k * k, dB.data_ptr(), M, M * N, BATCH))
+        X = dB.view(BATCH, N, M).transpose(1, 2)
+        B0 = torch.from_numpy(Bsrc[side]).cuda().view(BATCH, N, M).transpose(1, 2)
+        Am = dA.view(BATCH, k, k).transpose(1, 2)
+        Am = torch.nan_to_num(Am)
+        T = torch.triu(Am) if uplo == "U" else torch.tril(Am)
+        op = T.transpose(1, 2).contiguous() if trans != "N" else T.contiguous()
+        X = X.contiguous()
+        R = B0 - (torch.bmm(op, X) if side == "L" else torch.bmm(X, op))
+        n1 = lambda Z: Z.abs().sum(1).amax(1)
+        eta = n1(R) / (n1(op) * n1(X) + n1(B0))
+        worst = max(worst, float(eta.max()))
+    return worst
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the triangular-solve path is single-GPU here "
+              f"(--gpus {args.gpus} not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import rocblas_amd as rb
+    from rocblas_amd import synth
+
+    torch.cuda.set_device(0)
+    h = rb.Handle()
+    stream = torch.cuda.current_stream()
+    rb.check(h.set_stream(stream.cuda_stream))
+    data, Bsrc = make_inputs(torch, synth)
+
+    def step(events=None):
+        for idx, ((side, uplo, trans), (_, dA, dB)) in enumerate(data.items()):
+            k = M if side == "L" else N
+            if events is not None:
+                events[idx][0].record(stream)
+            st = rb.trsm_strided_batched(h, side, uplo, trans, "N", M, N, 1.0, dA.data_ptr(), k,
+                                         k * k, dB.data_ptr(), M, M * N, BATCH)
+            if events is not None:
+                events[idx][1].record(stream)
+            if st != rb.rocblas_status.success:
+                raise rb.RocblasError(st, "trsm_strided_batched")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # per-launch durations on the launch stream (each call is one kernel launch)
+    reps = max(10, min(args.steps, 50))
+    per_case = {c: [] for c in data}
+    for _ in range(reps):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in data]
+        step(ev)
+        torch.cuda.synchronize()
+        for idx, c in enumerate(data):
+            per_case[c].append(ev[idx][0].elapsed_time(ev[idx][1]) * 1e3)  # us
+    all_us = np.concatenate([np.array(v) for v in per_case.values()])
+    left_us = np.concatenate([np.array(v) for c, v in per_case.items() if c[0] == "L"])
+    right_us = np.concatenate([np.array(v) for c, v in per_case.items() if c[0] == "R"])
+    left_avg_s = float(left_us.mean()) * 1e-6
+    achieved = BATCH * algo_bytes("L") / left_avg_s / 1e9
+
+    eta = check_accuracy(torch, rb, h, data, Bsrc)
+    solves = len(CASES) * BATCH
+    flops = sum(BATCH * algo_flops(c[0]) for c in CASES)
+    out = {
+        "metric": "dtrsm solves/s (plain FP64, strided batched) at <=1e-12 backward error",
+        "value": solves / (ms_per_step * 1e-3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 seed 0)",
+        "config": {"workload": "dtrsm_strided_batched batch=20000 m=32 n=16, "
+                               "side/uplo/trans in {L,R}x{L,U}x{N,T}, diag=N, alpha=1",
+                   "batch": BATCH, "m": M, "n": N, "cases": 8},
+        "tflops": flops / (ms_per_step * 1e-3) / 1e12,
+        "max_backward_error": eta,
+        "latency_us": {"p50": float(np.percentile(all_us, 50)),
+                       "p99": float(np.percentile(all_us, 99)),
+                       "left_avg": float(left_us.mean()), "right_avg": float(right_us.mean())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "small_trsm_kernel<double,32> (side L)",
+                     "algorithmic_bytes_per_launch": BATCH * algo_bytes("L")},
+    }
+    if not args.no_cpu:
+        import oracle
+        oracle.build()
+        out["cpu_baseline"] = cpu_baseline(oracle, data, Bsrc)
+    h.close()
+    if eta > 1e-12:
+        print(f"backward error {eta:.3e} above 1e-12", file=sys.stderr)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

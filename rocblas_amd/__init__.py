@@ -1,0 +1,234 @@
+"""rocblas_amd -- MI355X-native triangular solve with rocBLAS's trsm C ABI.
+
+The product is ``librocblas_amd.so`` (built from ``csrc/`` by ``build.py``);
+this module binds its C entry points with ctypes under their rocBLAS names
+(``rocblas_dtrsm``, ``rocblas_dtrsm_strided_batched``, ...), with the same
+argument order, enum values and status codes as
+library/include/internal/rocblas-functions.h, so Python callers and tests read
+like the reference's own clients.  Device memory and streams come from
+PyTorch (plumbing only); pointers cross the ABI as plain integers.
+
+There is no fallback: if the shared library is missing, loading raises.
+"""
+import ctypes
+import enum
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librocblas_amd.so")
+
+
+class rocblas_status(enum.IntEnum):
+    success = 0
+    invalid_handle = 1
+    not_implemented = 2
+    invalid_pointer = 3
+    invalid_size = 4
+    memory_error = 5
+    internal_error = 6
+    perf_degraded = 7
+    size_query_mismatch = 8
+    size_increased = 9
+    size_unchanged = 10
+    invalid_value = 11
+    continue_ = 12
+    check_numerics_fail = 13
+    excluded_from_build = 14
+    arch_mismatch = 15
+
+
+class rocblas_operation(enum.IntEnum):
+    none = 111
+    transpose = 112
+    conjugate_transpose = 113
+
+
+class rocblas_fill(enum.IntEnum):
+    upper = 121
+    lower = 122
+    full = 123
+
+
+class rocblas_diagonal(enum.IntEnum):
+    non_unit = 131
+    unit = 132
+
+
+class rocblas_side(enum.IntEnum):
+    left = 141
+    right = 142
+    both = 143
+
+
+class rocblas_pointer_mode(enum.IntEnum):
+    host = 0
+    device = 1
+
+
+SIDE = {"L": rocblas_side.left, "R": rocblas_side.right}
+FILL = {"U": rocblas_fill.upper, "L": rocblas_fill.lower}
+OP = {"N": rocblas_operation.none, "T": rocblas_operation.transpose,
+      "C": rocblas_operation.conjugate_transpose}
+DIAG = {"N": rocblas_diagonal.non_unit, "U": rocblas_diagonal.unit}
+
+
+class RocblasError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = rocblas_status(status)
+        super().__init__(f"{what}: {self.status.name}")
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+
+# (name, restype, argtypes) for every symbol declared in include/rocblas_amd.h
+_TRSM_PLAIN = [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32]
+_TRSM_BATCHED = [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _i32]
+_TRSM_STRIDED = [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _i64, _vp, _i32, _i64,
+                 _i32]
+SYMBOLS = [
+    ("rocblas_create_handle", _i32, [ctypes.POINTER(_vp)]),
+    ("rocblas_destroy_handle", _i32, [_vp]),
+    ("rocblas_set_stream", _i32, [_vp, _vp]),
+    ("rocblas_get_stream", _i32, [_vp, ctypes.POINTER(_vp)]),
+    ("rocblas_set_pointer_mode", _i32, [_vp, _i32]),
+    ("rocblas_get_pointer_mode", _i32, [_vp, ctypes.POINTER(_i32)]),
+    ("rocblas_status_to_string", ctypes.c_char_p, [_i32]),
+    ("rocblas_start_device_memory_size_query", _i32, [_vp]),
+    ("rocblas_stop_device_memory_size_query", _i32, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    ("rocblas_is_device_memory_size_query", ctypes.c_bool, [_vp]),
+    ("rocblas_get_device_memory_size", _i32, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    ("rocblas_set_device_memory_size", _i32, [_vp, ctypes.c_size_t]),
+    ("rocblas_set_workspace", _i32, [_vp, _vp, ctypes.c_size_t]),
+    ("rocblas_is_managing_device_memory", ctypes.c_bool, [_vp]),
+    ("rocblas_is_user_managing_device_memory", ctypes.c_bool, [_vp]),
+    ("rocblas_strsm", _i32, _TRSM_PLAIN),
+    ("rocblas_dtrsm", _i32, _TRSM_PLAIN),
+    ("rocblas_strsm_batched", _i32, _TRSM_BATCHED),
+    ("rocblas_dtrsm_batched", _i32, _TRSM_BATCHED),
+    ("rocblas_strsm_strided_batched", _i32, _TRSM_STRIDED),
+    ("rocblas_dtrsm_strided_batched", _i32, _TRSM_STRIDED),
+]
+
+
+def lib():
+    """Load librocblas_amd.so (torch is imported first so both share one HIP runtime)."""
+    global _lib
+    if _lib is None:
+        try:
+            import torch  # noqa: F401  -- binds libamdhip64 before our library loads
+        except ImportError:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m rocblas_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SYMBOLS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def status_to_string(status):
+    return lib().rocblas_status_to_string(int(status)).decode()
+
+
+def check(status, what="rocblas call"):
+    if status != rocblas_status.success:
+        raise RocblasError(status, what)
+    return status
+
+
+class Handle:
+    """rocblas_handle owner (create on the current device, destroy on close)."""
+
+    def __init__(self):
+        h = _vp()
+        check(lib().rocblas_create_handle(ctypes.byref(h)), "rocblas_create_handle")
+        self.ptr = h.value
+
+    def close(self):
+        if self.ptr:
+            lib().rocblas_destroy_handle(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr):
+        return lib().rocblas_set_stream(self.ptr, stream_ptr)
+
+    def set_pointer_mode(self, mode):
+        return lib().rocblas_set_pointer_mode(self.ptr, int(mode))
+
+
+def _scalar(value, dtype_char):
+    if dtype_char == "d":
+        return ctypes.byref(ctypes.c_double(value)), ctypes.c_double(value)
+    return ctypes.byref(ctypes.c_float(value)), ctypes.c_float(value)
+
+
+def trsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, precision="d",
+         alpha_ptr=None):
+    """rocblas_{s,d}trsm on raw device pointers (ints).  ``alpha`` is a host
+    scalar unless ``alpha_ptr`` (a device pointer) is given.  Returns the status."""
+    fn = lib().rocblas_dtrsm if precision == "d" else lib().rocblas_strsm
+    a = alpha_ptr
+    keep = None
+    if a is None and alpha is not None:
+        keep = ctypes.c_double(alpha) if precision == "d" else ctypes.c_float(alpha)
+        a = ctypes.addressof(keep)
+    st = fn(_h(handle), int(SIDE.get(side, side)), int(FILL.get(uplo, uplo)),
+            int(OP.get(trans, trans)), int(DIAG.get(diag, diag)), m, n, a, A, lda, B, ldb)
+    del keep
+    return rocblas_status(st)
+
+
+def trsm_strided_batched(handle, side, uplo, trans, diag, m, n, alpha, A, lda, stride_a, B, ldb,
+                         stride_b, batch, precision="d", alpha_ptr=None):
+    fn = (lib().rocblas_dtrsm_strided_batched if precision == "d"
+          else lib().rocblas_strsm_strided_batched)
+    a = alpha_ptr
+    keep = None
+    if a is None and alpha is not None:
+        keep = ctypes.c_double(alpha) if precision == "d" else ctypes.c_float(alpha)
+        a = ctypes.addressof(keep)
+    st = fn(_h(handle), int(SIDE.get(side, side)), int(FILL.get(uplo, uplo)),
+            int(OP.get(trans, trans)), int(DIAG.get(diag, diag)), m, n, a, A, lda, stride_a, B,
+            ldb, stride_b, batch)
+    del keep
+    return rocblas_status(st)
+
+
+def trsm_batched(handle, side, uplo, trans, diag, m, n, alpha, A_array, lda, B_array, ldb, batch,
+                 precision="d", alpha_ptr=None):
+    """``A_array``/``B_array`` are device pointers to device arrays of pointers."""
+    fn = lib().rocblas_dtrsm_batched if precision == "d" else lib().rocblas_strsm_batched
+    a = alpha_ptr
+    keep = None
+    if a is None and alpha is not None:
+        keep = ctypes.c_double(alpha) if precision == "d" else ctypes.c_float(alpha)
+        a = ctypes.addressof(keep)
+    st = fn(_h(handle), int(SIDE.get(side, side)), int(FILL.get(uplo, uplo)),
+            int(OP.get(trans, trans)), int(DIAG.get(diag, diag)), m, n, a, A_array, lda, B_array,
+            ldb, batch)
+    del keep
+    return rocblas_status(st)
+
+
+def _h(handle):
+    if handle is None:
+        return None
+    return handle.ptr if isinstance(handle, Handle) else handle

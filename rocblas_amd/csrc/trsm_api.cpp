@@ -1,0 +1,202 @@
+// trsm_api.cpp -- extern "C" rocblas_{s,d}trsm{,_batched,_strided_batched}.
+//
+// Each entry point runs the reference's checks in the reference's order
+// (library/src/blas3/rocblas_trsm.hpp:29-82 via
+// library/src/blas3/rocblas_trsm_imp.hpp:54-157 and the batched /
+// strided-batched *_imp.hpp files), then hands a TrsmProblem to the launcher
+// in trsm_kernels.hip.  No exception crosses the ABI.
+#include "internal.hpp"
+
+#include <new>
+
+namespace rbamd
+{
+namespace
+{
+
+template <typename T>
+rocblas_status trsm_entry(rocblas_handle    handle,
+                          rocblas_side      side,
+                          rocblas_fill      uplo,
+                          rocblas_operation transA,
+                          rocblas_diagonal  diag,
+                          rocblas_int       m,
+                          rocblas_int       n,
+                          const T*          alpha,
+                          BatchPtr<const T> A,
+                          bool              A_null,
+                          rocblas_int       lda,
+                          BatchPtr<T>       B,
+                          bool              B_null,
+                          rocblas_int       ldb,
+                          rocblas_int       batch_count)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+
+    // enums (rocblas_trsm.hpp:37-51)
+    if(side != rocblas_side_left && side != rocblas_side_right)
+        return rocblas_status_invalid_value;
+    if(uplo != rocblas_fill_lower && uplo != rocblas_fill_upper)
+        return rocblas_status_invalid_value;
+    if(transA != rocblas_operation_none && transA != rocblas_operation_transpose
+       && transA != rocblas_operation_conjugate_transpose)
+        return rocblas_status_invalid_value;
+    if(diag != rocblas_diagonal_non_unit && diag != rocblas_diagonal_unit)
+        return rocblas_status_invalid_value;
+
+    // sizes and leading dimensions (rocblas_trsm.hpp:53-56)
+    const rocblas_int k = side == rocblas_side_left ? m : n;
+    if(batch_count < 0 || m < 0 || n < 0 || lda < k || ldb < m)
+        return rocblas_status_invalid_size;
+
+    // quick return (rocblas_trsm.hpp:58-61)
+    if(!m || !n || !batch_count)
+        return handle->size_query ? rocblas_status_size_unchanged : rocblas_status_success;
+
+    // pointers (rocblas_trsm.hpp:63-67)
+    if(!handle->size_query)
+    {
+        if(B_null || !alpha
+           || (handle->pointer_mode == rocblas_pointer_mode_host && *alpha != T(0) && A_null))
+            return rocblas_status_invalid_pointer;
+    }
+
+    TrsmProblem<T> p;
+    p.side  = side;
+    p.uplo  = uplo;
+    // conjugate transpose is transpose for real types
+    // (library/src/blas3/rocblas_trsm_kernels.hpp:3625-3626)
+    p.trans      = transA == rocblas_operation_none ? rocblas_operation_none
+                                                    : rocblas_operation_transpose;
+    p.diag       = diag;
+    p.m          = m;
+    p.n          = n;
+    p.lda        = lda;
+    p.ldb        = ldb;
+    p.A          = A;
+    p.B          = B;
+    p.batch      = batch_count;
+    p.alpha_host = T(1);
+    p.alpha_dev  = nullptr;
+
+    // device-memory size query: report, do no work
+    // (library/src/blas3/rocblas_trsm_kernels.hpp:1898-1910)
+    if(handle->size_query)
+    {
+        size_t need = roundup_ws(trsm_workspace_bytes(p));
+        if(!need || need <= handle->query_size)
+            return rocblas_status_size_unchanged;
+        handle->query_size = need;
+        return rocblas_status_size_increased;
+    }
+
+    if(handle->pointer_mode == rocblas_pointer_mode_host)
+    {
+        p.alpha_host = *alpha;
+        // alpha == 0: B := 0, A not referenced (rocblas_trsm_imp.hpp:154-157)
+        if(p.alpha_host == T(0))
+            return zero_matrix_launch<T>(handle, m, n, B, ldb, batch_count);
+    }
+    else
+    {
+        // alpha is read on the device by every kernel, so the call never
+        // synchronises and can be captured in a HIP graph.
+        p.alpha_dev = alpha;
+    }
+    return trsm_launch<T>(handle, p);
+}
+
+} // namespace
+} // namespace rbamd
+
+using namespace rbamd;
+
+#define RBAMD_TRY_BEGIN try {
+#define RBAMD_TRY_END                                 \
+    }                                                 \
+    catch(const std::bad_alloc&)                      \
+    {                                                 \
+        return rocblas_status_memory_error;           \
+    }                                                 \
+    catch(...)                                        \
+    {                                                 \
+        return rocblas_status_internal_error;         \
+    }
+
+#define RBAMD_TRSM_PLAIN(NAME, T)                                                               \
+    rocblas_status NAME(rocblas_handle    handle,                                               \
+                        rocblas_side      side,                                                 \
+                        rocblas_fill      uplo,                                                 \
+                        rocblas_operation transA,                                               \
+                        rocblas_diagonal  diag,                                                 \
+                        rocblas_int       m,                                                    \
+                        rocblas_int       n,                                                    \
+                        const T*          alpha,                                                \
+                        const T*          A,                                                    \
+                        rocblas_int       lda,                                                  \
+                        T*                B,                                                    \
+                        rocblas_int       ldb)                                                  \
+    {                                                                                           \
+        RBAMD_TRY_BEGIN                                                                         \
+        return trsm_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                     \
+                             BatchPtr<const T>{nullptr, A, 0, 0}, A == nullptr, lda,            \
+                             BatchPtr<T>{nullptr, B, 0, 0}, B == nullptr, ldb, 1);              \
+        RBAMD_TRY_END                                                                           \
+    }
+
+#define RBAMD_TRSM_BATCHED(NAME, T)                                                             \
+    rocblas_status NAME(rocblas_handle    handle,                                               \
+                        rocblas_side      side,                                                 \
+                        rocblas_fill      uplo,                                                 \
+                        rocblas_operation transA,                                               \
+                        rocblas_diagonal  diag,                                                 \
+                        rocblas_int       m,                                                    \
+                        rocblas_int       n,                                                    \
+                        const T*          alpha,                                                \
+                        const T* const    A[],                                                  \
+                        rocblas_int       lda,                                                  \
+                        T* const          B[],                                                  \
+                        rocblas_int       ldb,                                                  \
+                        rocblas_int       batch_count)                                          \
+    {                                                                                           \
+        RBAMD_TRY_BEGIN                                                                         \
+        return trsm_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                     \
+                             BatchPtr<const T>{A, nullptr, 0, 0}, A == nullptr, lda,            \
+                             BatchPtr<T>{B, nullptr, 0, 0}, B == nullptr, ldb, batch_count);    \
+        RBAMD_TRY_END                                                                           \
+    }
+
+#define RBAMD_TRSM_STRIDED(NAME, T)                                                             \
+    rocblas_status NAME(rocblas_handle    handle,                                               \
+                        rocblas_side      side,                                                 \
+                        rocblas_fill      uplo,                                                 \
+                        rocblas_operation transA,                                               \
+                        rocblas_diagonal  diag,                                                 \
+                        rocblas_int       m,                                                    \
+                        rocblas_int       n,                                                    \
+                        const T*          alpha,                                                \
+                        const T*          A,                                                    \
+                        rocblas_int       lda,                                                  \
+                        rocblas_stride    stride_a,                                             \
+                        T*                B,                                                    \
+                        rocblas_int       ldb,                                                  \
+                        rocblas_stride    stride_b,                                             \
+                        rocblas_int       batch_count)                                          \
+    {                                                                                           \
+        RBAMD_TRY_BEGIN                                                                         \
+        return trsm_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                     \
+                             BatchPtr<const T>{nullptr, A, stride_a, 0}, A == nullptr, lda,     \
+                             BatchPtr<T>{nullptr, B, stride_b, 0}, B == nullptr, ldb,           \
+                             batch_count);                                                      \
+        RBAMD_TRY_END                                                                           \
+    }
+
+extern "C" {
+RBAMD_TRSM_PLAIN(rocblas_strsm, float)
+RBAMD_TRSM_PLAIN(rocblas_dtrsm, double)
+RBAMD_TRSM_BATCHED(rocblas_strsm_batched, float)
+RBAMD_TRSM_BATCHED(rocblas_dtrsm_batched, double)
+RBAMD_TRSM_STRIDED(rocblas_strsm_strided_batched, float)
+RBAMD_TRSM_STRIDED(rocblas_dtrsm_strided_batched, double)
+}

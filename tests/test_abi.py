@@ -1,0 +1,93 @@
+"""CPU checks of the C ABI: the library loads, exports exactly what
+include/rocblas_amd.h declares, and its host-only paths (null handle, status
+strings) behave like the reference's (library/src/rocblas_auxiliary.cpp,
+library/src/handle.cpp, library/src/blas3/rocblas_trsm_imp.hpp:68-69)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import rocblas_amd as rb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rocblas_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"ROCBLAS_AMD_EXPORT[^;(]*?\b(rocblas_\w+)\s*\(", text)))
+
+
+def test_header_declares_the_trsm_path():
+    syms = declared_symbols()
+    for name in ["rocblas_dtrsm", "rocblas_strsm", "rocblas_dtrsm_batched",
+                 "rocblas_strsm_batched", "rocblas_dtrsm_strided_batched",
+                 "rocblas_strsm_strided_batched", "rocblas_create_handle",
+                 "rocblas_destroy_handle", "rocblas_set_stream", "rocblas_get_stream",
+                 "rocblas_set_pointer_mode", "rocblas_get_pointer_mode"]:
+        assert name in syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = rb.lib()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    # and the python binding table covers the header exactly
+    assert sorted(n for n, _, _ in rb.SYMBOLS) == declared_symbols()
+    out = subprocess.run(["nm", "-D", "--defined-only", rb.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = sorted(l.split()[-1] for l in out.splitlines() if " T " in l)
+    assert exported == declared_symbols()
+
+
+def test_header_compiles_as_c():
+    src = '#include "rocblas_amd.h"\nint main(void){ return (int)rocblas_status_success; }\n'
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.dirname(HEADER),
+                        "-x", "c", "-", "-o", "/dev/null"], input=src, text=True,
+                       capture_output=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_status_strings():
+    for s in rb.rocblas_status:
+        assert rb.status_to_string(s) == "rocblas_status_" + s.name.rstrip("_")
+    assert rb.status_to_string(99) == "<undefined rocblas_status value>"
+
+
+def test_null_handle_everywhere():
+    L = rb.lib()
+    IH = rb.rocblas_status.invalid_handle
+    assert L.rocblas_create_handle(None) == IH
+    assert L.rocblas_destroy_handle(None) == IH
+    assert L.rocblas_set_stream(None, None) == IH
+    assert L.rocblas_get_stream(None, None) == IH
+    assert L.rocblas_set_pointer_mode(None, 0) == IH
+    assert L.rocblas_get_pointer_mode(None, None) == IH
+    assert L.rocblas_start_device_memory_size_query(None) == IH
+    assert L.rocblas_stop_device_memory_size_query(None, None) == IH
+    assert L.rocblas_get_device_memory_size(None, None) == IH
+    assert L.rocblas_set_device_memory_size(None, 0) == IH
+    assert L.rocblas_set_workspace(None, None, 0) == IH
+    assert not L.rocblas_is_managing_device_memory(None)
+    assert not L.rocblas_is_user_managing_device_memory(None)
+    assert not L.rocblas_is_device_memory_size_query(None)
+    one = ctypes.c_double(1.0)
+    onef = ctypes.c_float(1.0)
+    args = (141, 121, 111, 131, 4, 4)
+    assert L.rocblas_dtrsm(None, *args, ctypes.addressof(one), 1, 4, 1, 4) == IH
+    assert L.rocblas_strsm(None, *args, ctypes.addressof(onef), 1, 4, 1, 4) == IH
+    assert L.rocblas_dtrsm_batched(None, *args, ctypes.addressof(one), 1, 4, 1, 4, 1) == IH
+    assert L.rocblas_dtrsm_strided_batched(None, *args, ctypes.addressof(one), 1, 4, 16, 1, 4,
+                                           16, 1) == IH
+
+
+def test_create_handle_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    st = rb.lib().rocblas_create_handle(ctypes.byref(h))
+    assert st != rb.rocblas_status.success
+    assert h.value is None

@@ -1,0 +1,405 @@
+"""GPU parity tests of rocblas_{s,d}trsm{,_batched,_strided_batched} through the C ABI.
+
+Mirrors the reference's test strategy (clients/include/blas3/testing_trsm*.hpp,
+clients/gtest/trsm_gtest.yaml): the bad-argument status table, quick returns,
+alpha == 0, the known-X forward-error/residual check at 40*eps*M, batched and
+strided-batched forms, repeatability and graph capture -- plus exact checks the
+reference lacks: exact-rational golden vectors and bit-exact integer problems.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import rocblas_amd as rb
+from rocblas_amd import synth
+from tests.gpu_util import (GpuHandle, dev, eta_torch, gpu_trsm, host, logical, ptr_array, say)
+from tests.helpers import ALL_CASES, integer_kat, load_rational_cases
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = np.finfo(np.float64).eps
+EPS32 = np.finfo(np.float32).eps
+S = rb.rocblas_status
+
+
+@pytest.fixture(scope="module")
+def h():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    rb.lib()
+    oracle.build()
+    hd = GpuHandle()
+    yield hd
+    hd.close()
+
+
+# ---------------------------------------------------------------- statuses --
+
+@pytest.mark.parametrize("mode", ["host", "device"])
+def test_bad_arg_table(h, mode):
+    """clients/include/blas3/testing_trsm.hpp:33-186, both pointer modes."""
+    say("bad args", mode)
+    M = N = lda = ldb = 100
+    dA = torch.zeros(lda * M, dtype=torch.float64, device="cuda")
+    dB = torch.zeros(ldb * N, dtype=torch.float64, device="cuda")
+    one_h, zero_h = ctypes.c_double(1.0), ctypes.c_double(0.0)
+    one_d = torch.ones(1, dtype=torch.float64, device="cuda")
+    zero_d = torch.zeros(1, dtype=torch.float64, device="cuda")
+    h.set_pointer_mode(rb.rocblas_pointer_mode.device if mode == "device"
+                       else rb.rocblas_pointer_mode.host)
+    try:
+        alpha = one_d.data_ptr() if mode == "device" else ctypes.addressof(one_h)
+        zero = zero_d.data_ptr() if mode == "device" else ctypes.addressof(zero_h)
+        L = rb.lib()
+        A, B = dA.data_ptr(), dB.data_ptr()
+        s, u, t, d = 141, 121, 111, 131
+
+        def call(*a):
+            r = L.rocblas_dtrsm(*a)
+            torch.cuda.synchronize()
+            return r
+
+        assert call(h.ptr, 143, u, t, d, M, N, alpha, A, lda, B, ldb) == S.invalid_value
+        assert call(h.ptr, s, 143, t, d, M, N, alpha, A, lda, B, ldb) == S.invalid_value
+        assert call(h.ptr, s, u, 143, d, M, N, alpha, A, lda, B, ldb) == S.invalid_value
+        assert call(h.ptr, s, u, t, 143, M, N, alpha, A, lda, B, ldb) == S.invalid_value
+        assert call(h.ptr, s, u, t, d, -1, N, alpha, A, lda, B, ldb) == S.invalid_size
+        assert call(h.ptr, s, u, t, d, M, -1, alpha, A, lda, B, ldb) == S.invalid_size
+        assert call(h.ptr, s, u, t, d, M, N, alpha, A, lda, B, M - 1) == S.invalid_size
+        assert call(h.ptr, 141, u, t, d, M, N, alpha, A, M - 1, B, ldb) == S.invalid_size
+        assert call(h.ptr, 142, u, t, d, M, N, alpha, A, N - 1, B, ldb) == S.invalid_size
+        assert call(None, s, u, t, d, M, N, alpha, A, lda, B, ldb) == S.invalid_handle
+        assert call(h.ptr, s, u, t, d, M, N, None, A, lda, B, ldb) == S.invalid_pointer
+        if mode == "host":
+            assert call(h.ptr, s, u, t, d, M, N, alpha, None, lda, B, ldb) == S.invalid_pointer
+        assert call(h.ptr, s, u, t, d, M, N, alpha, A, lda, None, ldb) == S.invalid_pointer
+        # alpha == 0: A may be null, B := 0
+        dB.fill_(7.0)
+        assert call(h.ptr, s, u, t, d, M, N, zero, None, lda, B, ldb) == S.success
+        assert float(dB.abs().max()) == 0.0
+        # m == 0 / n == 0: every pointer may be null
+        assert call(h.ptr, s, u, t, d, 0, N, None, None, lda, None, ldb) == S.success
+        assert call(h.ptr, s, u, t, d, M, 0, None, None, lda, None, ldb) == S.success
+        # batched forms: batch_count < 0 is invalid, == 0 is a quick return
+        assert L.rocblas_dtrsm_strided_batched(h.ptr, s, u, t, d, M, N, alpha, A, lda, 0, B,
+                                               ldb, 0, -1) == S.invalid_size
+        assert L.rocblas_dtrsm_strided_batched(h.ptr, s, u, t, d, M, N, None, None, lda, 0,
+                                               None, ldb, 0, 0) == S.success
+        assert L.rocblas_dtrsm_batched(h.ptr, s, u, t, d, M, N, alpha, None, lda, None, ldb,
+                                       0) == S.success
+        assert L.rocblas_dtrsm_batched(h.ptr, s, u, t, d, M, N, alpha, A, lda, None, ldb,
+                                       1) == S.invalid_pointer
+    finally:
+        h.set_pointer_mode(rb.rocblas_pointer_mode.host)
+
+
+def test_handle_services(h):
+    L = rb.lib()
+    sz = ctypes.c_size_t()
+    assert L.rocblas_get_device_memory_size(h.ptr, ctypes.byref(sz)) == S.success
+    assert sz.value == 128 << 20
+    assert L.rocblas_is_managing_device_memory(h.ptr)
+    mode = ctypes.c_int32(-1)
+    assert L.rocblas_get_pointer_mode(h.ptr, ctypes.byref(mode)) == S.success
+    assert mode.value == 0
+    st = ctypes.c_void_p()
+    assert L.rocblas_get_stream(h.ptr, ctypes.byref(st)) == S.success
+    assert (st.value or 0) == torch.cuda.current_stream().cuda_stream
+    # size-query protocol (library/src/handle.cpp:449-483): trsm reports, does no work
+    assert L.rocblas_stop_device_memory_size_query(h.ptr, ctypes.byref(sz)) \
+        == S.size_query_mismatch
+    assert L.rocblas_start_device_memory_size_query(h.ptr) == S.success
+    assert L.rocblas_start_device_memory_size_query(h.ptr) == S.size_query_mismatch
+    assert L.rocblas_is_device_memory_size_query(h.ptr)
+    dB = torch.full((64 * 64,), 3.0, dtype=torch.float64, device="cuda")
+    st2 = rb.trsm(h, "L", "U", "N", "N", 64, 64, 1.0, None, 64, dB.data_ptr(), 64)
+    assert st2 in (S.size_unchanged, S.size_increased)
+    st2 = rb.trsm(h, "L", "U", "N", "N", 4096, 4096, 1.0, None, 4096, None, 4096)
+    assert st2 in (S.size_unchanged, S.size_increased)
+    assert L.rocblas_stop_device_memory_size_query(h.ptr, ctypes.byref(sz)) == S.success
+    torch.cuda.synchronize()
+    assert float(dB.min()) == 3.0 and float(dB.max()) == 3.0  # untouched
+    # user-managed / user-owned workspace and back
+    h2 = GpuHandle()
+    assert L.rocblas_set_device_memory_size(h2.ptr, 1 << 20) == S.success
+    assert L.rocblas_is_user_managing_device_memory(h2.ptr)
+    buf = torch.empty(1 << 20, dtype=torch.uint8, device="cuda")
+    assert L.rocblas_set_workspace(h2.ptr, buf.data_ptr(), buf.numel()) == S.success
+    assert not L.rocblas_is_managing_device_memory(h2.ptr)
+    assert L.rocblas_set_workspace(h2.ptr, None, 0) == S.success
+    assert L.rocblas_is_managing_device_memory(h2.ptr)
+    h2.close()
+
+
+# ------------------------------------------------------------ exact checks --
+
+def test_golden_rational_vectors(h):
+    cases = load_rational_cases()
+    say("golden rational", len(cases))
+    for c in cases:
+        m, n = c["m"], c["n"]
+        st, X = gpu_trsm(h, c["side"], c["uplo"], c["trans"], c["diag"], m, n, c["alpha"],
+                         c["A"], c["B"])
+        assert st == S.success
+        got, want = X[:m, :n], c["X"][:m, :n]
+        assert np.all(np.isfinite(got)), c
+        scale = max(1.0, np.abs(want).max())
+        assert np.abs(got - want).max() <= 64 * EPS64 * scale, (c["side"], c["uplo"],
+                                                               c["trans"], c["diag"], m, n)
+        if c["ldb"] > m:
+            assert np.all(np.isnan(X[m:, :])), "rows past m must be untouched"
+
+
+@pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
+def test_integer_kat_bitwise(h, side, uplo, trans):
+    """Exactly solvable problems: every intermediate of substitution and of the
+    GEMM updates is an exact small integer or dyadic, so X is reproduced bit for bit
+    on both the small kernel (k <= 64) and the recursive blocked path."""
+    rng = np.random.default_rng(101)
+    for diag in "NU":
+        for (m, n) in [(1, 1), (5, 3), (17, 9), (33, 40), (64, 16), (65, 7), (130, 33),
+                       (200, 150), (300, 64)]:
+            for alpha in (1.0, -0.5):
+                say("kat", side, uplo, trans, diag, m, n, alpha)
+                k = m if side == "L" else n
+                A, B, X = integer_kat(side, uplo, trans, diag, m, n, rng, lda=k + 2, ldb=m + 3,
+                                      alpha=alpha)
+                st, got = gpu_trsm(h, side, uplo, trans, diag, m, n, alpha, A, B)
+                assert st == S.success
+                assert np.array_equal(got[:m, :n], X[:m, :n]), (diag, m, n, alpha)
+
+
+# ------------------------------------------------- reference's acceptance --
+
+@pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
+def test_reference_known_x_procedure(h, side, uplo, trans):
+    """clients/include/blas3/testing_trsm.hpp:297-315, 433-448: diagonally dominant A
+    (unit-diagonal variant via make_unit_diagonal), random X, B := op(A) X / alpha,
+    forward error and residual <= 40*eps*M."""
+    rng = np.random.default_rng(202)
+    for (m, n) in [(7, 5), (32, 16), (64, 64), (65, 130), (257, 100), (600, 300)]:
+        for diag in "NU":
+            say("known-x", side, uplo, trans, diag, m, n)
+            alpha = 2.5
+            k = m if side == "L" else n
+            A = oracle.diag_dominant_triangular(k, uplo, rng, lda=k + 1)
+            if diag == "U":
+                oracle.make_unit_diagonal(A, uplo, k)
+            X = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+            B = X.copy(order="F")
+            oracle.trmm(side, uplo, trans, diag, m, n, 1.0 / alpha, A, B)
+            st, got = gpu_trsm(h, side, uplo, trans, diag, m, n, alpha, A, B)
+            assert st == S.success
+            fwd = oracle.norm1_rel_diff(m, n, got, X)
+            assert fwd <= 40 * EPS64 * m, fwd
+            R = np.asfortranarray(got[:m, :n])
+            oracle.trmm(side, uplo, trans, diag, m, n, 1.0 / alpha, A, R)
+            assert oracle.norm1_rel_diff(m, n, R, B) <= 40 * EPS64 * m
+
+
+@pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
+def test_float32(h, side, uplo, trans):
+    rng = np.random.default_rng(303)
+    for (m, n) in [(16, 16), (48, 20), (100, 70), (300, 129)]:
+        say("strsm", side, uplo, trans, m, n)
+        k = m if side == "L" else n
+        A = oracle.diag_dominant_triangular(k, uplo, rng)
+        X = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        B = X.copy(order="F")
+        oracle.trmm(side, uplo, trans, "N", m, n, 1.0, A, B)
+        A32 = np.asfortranarray(A.astype(np.float32))
+        B32 = np.asfortranarray(B.astype(np.float32))
+        st, got = gpu_trsm(h, side, uplo, trans, "N", m, n, 1.0, A32, B32, precision="s")
+        assert st == S.success
+        got64 = np.asfortranarray(got.astype(np.float64))
+        assert oracle.norm1_rel_diff(m, n, got64, X) <= 40 * EPS32 * m
+        eta = oracle.backward_error(side, uplo, trans, "N", m, n, 1.0, A32, B32, got)
+        assert eta <= 1e-5
+
+
+# --------------------------------------------------------- batched forms --
+
+@pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
+def test_strided_and_pointer_batched_match_single(h, side, uplo, trans):
+    rng = np.random.default_rng(404)
+    for (m, n, batch) in [(32, 16, 7), (17, 33, 3), (100, 20, 5), (1, 9, 4)]:
+        say("batched", side, uplo, trans, m, n, batch)
+        k = m if side == "L" else n
+        lda, ldb = k + 1, m + 2
+        sa, sb = lda * k + 3, ldb * n + 5
+        A = np.full(sa * batch, np.nan)
+        B = rng.uniform(-1, 1, sb * batch)
+        for b in range(batch):
+            Ab = oracle.diag_dominant_triangular(k, uplo, rng, lda=lda)
+            A[b * sa:b * sa + lda * k] = Ab.ravel(order="F")
+        dA = torch.from_numpy(A).cuda()
+        dB = torch.from_numpy(B.copy()).cuda()
+        st = rb.trsm_strided_batched(h, side, uplo, trans, "N", m, n, 0.5, dA.data_ptr(), lda, sa,
+                                     dB.data_ptr(), ldb, sb, batch)
+        assert st == S.success
+        # pointer-array form on separate copies
+        As = [torch.from_numpy(A[b * sa:b * sa + lda * k].copy()).cuda() for b in range(batch)]
+        Bs = [torch.from_numpy(B[b * sb:b * sb + ldb * n].copy()).cuda() for b in range(batch)]
+        pa, pb = ptr_array(As), ptr_array(Bs)
+        st = rb.trsm_batched(h, side, uplo, trans, "N", m, n, 0.5, pa.data_ptr(), lda,
+                             pb.data_ptr(), ldb, batch)
+        assert st == S.success
+        torch.cuda.synchronize()
+        Bout = dB.cpu().numpy()
+        for b in range(batch):
+            Ab = np.asfortranarray(A[b * sa:b * sa + lda * k].reshape(k, lda).T)
+            Bb = np.asfortranarray(B[b * sb:b * sb + ldb * n].reshape(n, ldb).T)
+            st1, single = gpu_trsm(h, side, uplo, trans, "N", m, n, 0.5, Ab, Bb)
+            got = np.asfortranarray(Bout[b * sb:b * sb + ldb * n].reshape(n, ldb).T)
+            assert np.array_equal(got, single), b
+            assert np.array_equal(host(Bs[b], (ldb, n)), single), b
+            # gaps between instances untouched
+            assert np.array_equal(Bout[b * sb + ldb * n:(b + 1) * sb],
+                                  B[b * sb + ldb * n:(b + 1) * sb])
+            ref = Bb.copy(order="F")
+            oracle.trsm(side, uplo, trans, "N", m, n, 0.5, Ab, ref)
+            assert oracle.norm1_rel_diff(m, n, got, ref) <= 40 * EPS64 * max(m, n)
+
+
+# ---------------------------------------------------- pointer mode / alpha --
+
+@pytest.mark.parametrize("m,n", [(40, 24), (200, 90)])
+def test_device_pointer_mode_matches_host(h, m, n):
+    rng = np.random.default_rng(505)
+    for side, uplo, trans in ALL_CASES:
+        say("device alpha", side, uplo, trans, m, n)
+        k = m if side == "L" else n
+        A = oracle.diag_dominant_triangular(k, uplo, rng)
+        B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        _, want = gpu_trsm(h, side, uplo, trans, "N", m, n, -1.5, A, B)
+        dA, dB = dev(A), dev(B)
+        alpha_d = torch.tensor([-1.5], dtype=torch.float64, device="cuda")
+        h.set_pointer_mode(rb.rocblas_pointer_mode.device)
+        try:
+            st = rb.trsm(h, side, uplo, trans, "N", m, n, None, dA.data_ptr(), k, dB.data_ptr(), m,
+                         alpha_ptr=alpha_d.data_ptr())
+            assert st == S.success
+            torch.cuda.synchronize()
+            assert np.array_equal(host(dB, (m, n)), want)
+            # device alpha == 0 with A == nullptr: B := 0, A never read
+            alpha_d.zero_()
+            st = rb.trsm(h, side, uplo, trans, "N", m, n, None, None, k, dB.data_ptr(), m,
+                         alpha_ptr=alpha_d.data_ptr())
+            assert st == S.success
+            torch.cuda.synchronize()
+            assert float(dB.abs().max()) == 0.0
+        finally:
+            h.set_pointer_mode(rb.rocblas_pointer_mode.host)
+
+
+def test_unreferenced_memory_untouched_and_unread(h):
+    """NaN in the other triangle, on a unit diagonal and in B's ldb padding never
+    reaches the solution, and the padding is left as it was."""
+    rng = np.random.default_rng(606)
+    for side, uplo, trans in ALL_CASES:
+        for (m, n) in [(20, 12), (150, 70)]:
+            k = m if side == "L" else n
+            A = oracle.diag_dominant_triangular(k, uplo, rng, lda=k + 3)  # NaN elsewhere
+            oracle.make_unit_diagonal(A, uplo, k)  # NaN diagonal, diag=U
+            B = np.full((m + 4, n), np.nan, order="F")
+            B[:m] = rng.uniform(-1, 1, (m, n))
+            st, got = gpu_trsm(h, side, uplo, trans, "U", m, n, 1.0, A, B)
+            assert st == S.success
+            assert np.all(np.isfinite(got[:m])) and np.all(np.isnan(got[m:]))
+
+
+def test_repeatability_bitwise(h):
+    """clients/gtest/trsm_gtest.yaml:1310-1334: identical results run to run."""
+    rng = np.random.default_rng(707)
+    for (m, n) in [(32, 16), (500, 300)]:
+        A = synth.tri_uniform(m, "U", 1)
+        B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        outs = [gpu_trsm(h, "L", "U", "T", "N", m, n, 1.0, A, B)[1] for _ in range(3)]
+        assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def test_graph_capture_replays_match_eager(h):
+    """Host-pointer-mode calls captured into a HIP graph replay to the eager result
+    (clients/common/client_utility.cpp:354-385), with eager work between replays."""
+    rng = np.random.default_rng(808)
+    m, n = 300, 100
+    A = synth.tri_uniform(m, "L", 3)
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    _, want = gpu_trsm(h, "L", "L", "N", "N", m, n, 2.0, A, B0)
+    dA, dB0 = dev(A), dev(B0)
+    dB = dB0.clone()
+    s = torch.cuda.Stream()
+    hg = GpuHandle()
+    rb.check(hg.set_stream(s.cuda_stream))
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        st = rb.trsm(hg, "L", "L", "N", "N", m, n, 2.0, dA.data_ptr(), m, dB.data_ptr(), m)
+    assert st == S.success
+    for rep in range(4):
+        dB.copy_(dB0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(dB, (m, n)), want), rep
+        _ = (dB0 * 2).sum().item()  # eager work between replays
+    hg.close()
+
+
+# --------------------------------------------------------- workload sizes --
+
+def test_batched_workload_size_all_cases(h):
+    """The strided-batched workload: 20000 instances, m=32, n=16, every side/uplo/trans
+    case, A = (diag U[1,2], off-diagonal U[-1,1]/m) / m, B = U[-1,1]; per-instance
+    backward error and agreement with the CPU oracle."""
+    batch, m, n = 20000, 32, 16
+    for side, uplo, trans in ALL_CASES:
+        say("workload batch", side, uplo, trans)
+        k = m if side == "L" else n
+        A = synth.tri_uniform(k, uplo, 0, batch=batch, scale=1.0 / k)
+        Bm = synth.rhs_uniform(m, n, 0, batch=batch)
+        Af = np.ascontiguousarray(np.swapaxes(A, 1, 2)).ravel()
+        Bf = np.ascontiguousarray(np.swapaxes(Bm, 1, 2)).ravel()
+        dA = torch.from_numpy(Af).cuda()
+        dB = torch.from_numpy(Bf.copy()).cuda()
+        st = rb.trsm_strided_batched(h, side, uplo, trans, "N", m, n, 1.0, dA.data_ptr(), k, k * k,
+                                     dB.data_ptr(), m, m * n, batch)
+        assert st == S.success
+        got = dB.cpu().numpy()
+        ref = Bf.copy()
+        oracle.trsm_batched_omp(side, uplo, trans, "N", m, n, 1.0, Af, k, k * k, ref, m, m * n,
+                                batch)
+        Xg = got.reshape(batch, n, m).transpose(0, 2, 1)
+        Xr = ref.reshape(batch, n, m).transpose(0, 2, 1)
+        rel = np.abs(Xg - Xr).max(axis=(1, 2)) / np.abs(Xr).max(axis=(1, 2))
+        assert rel.max() <= 1e-12
+        # per-instance backward error, vectorised float64 (residual is O(m^2 n))
+        Ad = A[:, :k, :k].copy()
+        Ad = np.where(np.isnan(Ad), 0.0, Ad)
+        Ad = np.triu(Ad) if uplo == "U" else np.tril(Ad)
+        op = np.swapaxes(Ad, 1, 2) if trans != "N" else Ad
+        B0 = Bm[:, :m, :n]
+        R = B0 - (op @ Xg if side == "L" else Xg @ op)
+        n1 = lambda M: np.abs(M).sum(axis=1).max(axis=1)
+        eta = n1(R) / (n1(op) * n1(Xg) + n1(B0))
+        assert eta.max() <= 1e-14
+
+
+def test_large_single_solve_backward_error(h):
+    """m = n = 4096, upper, N(0,1)/sqrt(m) off-diagonal (condition number ~1e3):
+    backward error from a plain PyTorch float64 residual on the device."""
+    m = n = 4096
+    say("large", m)
+    A = synth.tri_normal(m, "U", 0)
+    Bm = synth.rhs_uniform(m, n, 0)
+    dA, dB0 = dev(A), dev(Bm)
+    dB = dB0.clone()
+    st = rb.trsm(h, "L", "U", "N", "N", m, n, 1.0, dA.data_ptr(), m, dB.data_ptr(), m)
+    assert st == S.success
+    torch.cuda.synchronize()
+    eta = eta_torch("L", "U", "N", "N", 1.0, logical(dA, m, m, m), logical(dB0, m, m, n),
+                    logical(dB, m, m, n))
+    say("large eta", eta)
+    assert eta <= 1e-14
