@@ -1,0 +1,142 @@
+// device_common.hpp -- shared device-side definitions: MFMA wrappers, kernel
+// argument blocks, and the launch interfaces between the driver and the
+// kernel translation units.
+#pragma once
+
+#include "internal.hpp"
+
+#include <algorithm>
+
+namespace rbamd
+{
+constexpr int WAVE = 64;
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float  f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+
+// Operand layout of both 16x16x4 forms: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16].  Result layout differs (measured on gfx950): element r of the
+// f64 result in lane l is D[l/16 + 4r][l%16]; of the f32 result, D[4(l/16) + r][l%16].
+template <>
+struct Mfma<double>
+{
+    using acc_t = d4_t;
+    static __device__ __forceinline__ acc_t run(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r)
+    {
+        return (lane >> 4) + 4 * r;
+    }
+};
+
+template <>
+struct Mfma<float>
+{
+    using acc_t = f4_t;
+    static __device__ __forceinline__ acc_t run(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r)
+    {
+        return 4 * (lane >> 4) + r;
+    }
+};
+
+inline rocblas_status launch_status()
+{
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? rocblas_status_success : hip_to_status(e);
+}
+
+inline unsigned grid_batch(int batch)
+{
+    return unsigned(std::min(batch, 65535));
+}
+
+// ---------------------------------------------------------------------------
+// Small solve (k <= 64)
+// ---------------------------------------------------------------------------
+enum SmallFlags : int
+{
+    SF_TA    = 1,  // M[i][j] = A[j][i]
+    SF_UPPER = 2,  // stored triangle of A is the upper one
+    SF_REV   = 4,  // canonical index i is real index K-1-i
+    SF_UNIT  = 8,  // unit diagonal: the diagonal of A is never read
+    SF_COLS  = 16, // the vectors are columns of B (side left)
+    SF_SCALE = 32  // multiply the right-hand side by alpha
+};
+
+template <typename T>
+struct SmallArgs
+{
+    BatchPtr<const T> A;
+    BatchPtr<T>       B;
+    int               K;
+    int               nvec;
+    int               lda;
+    int               ldb;
+    int               vp_log2;     // vectors per slot = 1 << vp_log2
+    int               chunks;      // ceil(nvec / vectors per slot)
+    int64_t           units;       // batch * chunks
+    int               flags;
+    int               slot_stride; // elements per slot of the packed triangle in LDS
+    int               bs_stride;   // elements per staged vector (side L)
+    T                 alpha_host;
+    const T*          alpha_dev;
+};
+
+// Largest number of instance slots per wavefront for a given KMAX: keeps the
+// per-lane share of the slots' triangles at <= ~37 elements (all of them are
+// loaded before the first wait) and the LDS region near 17-25 KiB.
+constexpr int small_gmax(int kmax)
+{
+    return kmax <= 8 ? 64 : kmax <= 16 ? 16 : kmax <= 32 ? 4 : kmax <= 48 ? 2 : 1;
+}
+
+// packed canonical lower layout: element (i, j), i >= j, at colstart(j) + i - j
+template <int KMAX>
+__host__ __device__ constexpr int colstart(int j)
+{
+    return j * KMAX - j * (j - 1) / 2;
+}
+
+constexpr int slot_elems(int kmax)
+{
+    // odd, so consecutive slots start in different LDS banks
+    return (kmax * (kmax + 1) / 2) | 1;
+}
+
+// Launch one small-solve grid; defined per KMAX in small_k*.hip.
+template <typename T, int KMAX>
+rocblas_status launch_small_kmax(rocblas_handle h, const SmallArgs<T>& a, dim3 grid, size_t lds);
+
+// ---------------------------------------------------------------------------
+// GEMM (trailing updates)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct GemmArgs
+{
+    int               M, N, K;
+    int               opA, opB; // 0 = none, 1 = transpose
+    BatchPtr<const T> A;
+    int               lda;
+    BatchPtr<const T> B;
+    int               ldb;
+    BatchPtr<T>       C;
+    int               ldc;
+    T                 alpha, beta;
+    const T*          skip_if_zero; // device scalar; == 0 -> no work (alpha == 0 in trsm)
+    int               tiles_m, tiles_n;
+    int               batch;
+};
+
+template <typename T>
+rocblas_status launch_gemm(rocblas_handle h, const GemmArgs<T>& a);
+
+} // namespace rbamd

@@ -1,0 +1,260 @@
+// gemm.hip -- MFMA GEMM for the trailing updates of the blocked solve.
+//
+//   C := alpha * op(A) * op(B) + beta * C   (column-major, batched)
+//
+// FP64 uses v_mfma_f64_16x16x4_f64, FP32 v_mfma_f32_16x16x4_f32.  Tiles are
+// staged through LDS (double buffered), with the next K step's global loads in
+// flight during the current step's MFMAs.  Workgroup ids are remapped so that
+// each XCD's L2 sees contiguous runs of tiles, grouped by row panel.
+#include "device_common.hpp"
+
+namespace rbamd
+{
+// ---------------------------------------------------------------------------
+// MFMA GEMM:  C := alpha * op(A) * op(B) + beta * C   (column-major, batched)
+//   op(A) is M x K, op(B) is K x N.  Workgroup tile BM x BN, K step BK,
+//   WGM x WGN wavefronts each owning a (BM/WGM) x (BN/WGN) block of 16x16
+//   MFMA tiles.  Global -> register prefetch of the next K step overlaps the
+//   MFMAs of the current one; LDS is double buffered (one barrier per step).
+// ---------------------------------------------------------------------------
+
+template <typename T, int BM, int BN, int BK, int WGM, int WGN>
+__global__ void __launch_bounds__(WAVE * WGM * WGN) gemm_kernel(const GemmArgs<T> a)
+{
+    using acc_t              = typename Mfma<T>::acc_t;
+    constexpr int NT         = WAVE * WGM * WGN;
+    constexpr int WTM        = BM / WGM;
+    constexpr int WTN        = BN / WGN;
+    constexpr int TM         = WTM / 16;
+    constexpr int TN         = WTN / 16;
+    constexpr int LDA_S      = BM + 2;
+    constexpr int LDB_S      = BN + 2;
+    constexpr int EA         = BM * BK / NT;
+    constexpr int EB         = BK * BN / NT;
+    static_assert(EA * NT == BM * BK && EB * NT == BK * BN, "tile/threads mismatch");
+    static_assert(TM >= 1 && TN >= 1 && BK % 4 == 0, "bad tile");
+
+    __shared__ T As[2][BK * LDA_S];
+    __shared__ T Bs[2][BK * LDB_S];
+
+    if(a.skip_if_zero && *a.skip_if_zero == T(0))
+        return;
+
+    const int tid  = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int w    = tid / WAVE;
+    const int wm   = w % WGM;
+    const int wn   = w / WGM;
+
+    // tile index: spread consecutive tiles over the 8 XCDs' L2s in contiguous
+    // runs, then group row-panels so neighbouring workgroups share A panels.
+    const int total = a.tiles_m * a.tiles_n;
+    int       bid   = blockIdx.x;
+    {
+        const int q = total / 8, r = total % 8;
+        const int x = bid % 8, l = bid / 8;
+        bid         = x * q + min(x, r) + l;
+    }
+    constexpr int GROUP = 8;
+    const int     grp   = bid / (GROUP * a.tiles_n);
+    const int     fm    = grp * GROUP;
+    const int     gsz   = min(a.tiles_m - fm, GROUP);
+    const int     tm    = fm + (bid % gsz);
+    const int     tn    = (bid % (GROUP * a.tiles_n)) / gsz;
+    const int     m0    = tm * BM;
+    const int     n0    = tn * BN;
+
+    for(int b = blockIdx.y; b < a.batch; b += gridDim.y)
+    {
+        const T* Ap = a.A.get(b);
+        const T* Bp = a.B.get(b);
+        T*       Cp = a.C.get(b);
+
+        acc_t acc[TM][TN];
+#pragma unroll
+        for(int i = 0; i < TM; ++i)
+#pragma unroll
+            for(int j = 0; j < TN; ++j)
+                acc[i][j] = acc_t{0, 0, 0, 0};
+
+        T ra[EA], rb[EB];
+
+        auto load_tiles = [&](int k0) {
+#pragma unroll
+            for(int s = 0; s < EA; ++s)
+            {
+                const int e = tid + NT * s;
+                int       i, kk;
+                if(a.opA == 0)
+                {
+                    i  = e % BM;
+                    kk = e / BM;
+                }
+                else
+                {
+                    kk = e % BK;
+                    i  = e / BK;
+                }
+                const int gi = m0 + i, gk = k0 + kk;
+                T         v  = T(0);
+                if(gi < a.M && gk < a.K)
+                    v = a.opA == 0 ? Ap[gi + int64_t(gk) * a.lda] : Ap[gk + int64_t(gi) * a.lda];
+                ra[s] = v;
+            }
+#pragma unroll
+            for(int s = 0; s < EB; ++s)
+            {
+                const int e = tid + NT * s;
+                int       j, kk;
+                if(a.opB == 0)
+                {
+                    kk = e % BK;
+                    j  = e / BK;
+                }
+                else
+                {
+                    j  = e % BN;
+                    kk = e / BN;
+                }
+                const int gj = n0 + j, gk = k0 + kk;
+                T         v  = T(0);
+                if(gj < a.N && gk < a.K)
+                    v = a.opB == 0 ? Bp[gk + int64_t(gj) * a.ldb] : Bp[gj + int64_t(gk) * a.ldb];
+                rb[s] = v;
+            }
+        };
+        auto store_tiles = [&](int buf) {
+#pragma unroll
+            for(int s = 0; s < EA; ++s)
+            {
+                const int e = tid + NT * s;
+                int       i, kk;
+                if(a.opA == 0)
+                {
+                    i  = e % BM;
+                    kk = e / BM;
+                }
+                else
+                {
+                    kk = e % BK;
+                    i  = e / BK;
+                }
+                As[buf][kk * LDA_S + i] = ra[s];
+            }
+#pragma unroll
+            for(int s = 0; s < EB; ++s)
+            {
+                const int e = tid + NT * s;
+                int       j, kk;
+                if(a.opB == 0)
+                {
+                    kk = e % BK;
+                    j  = e / BK;
+                }
+                else
+                {
+                    j  = e % BN;
+                    kk = e / BN;
+                }
+                Bs[buf][kk * LDB_S + j] = rb[s];
+            }
+        };
+
+        const int ktiles = (a.K + BK - 1) / BK;
+        load_tiles(0);
+        for(int kt = 0; kt < ktiles; ++kt)
+        {
+            const int buf = kt & 1;
+            store_tiles(buf);
+            __syncthreads();
+            if(kt + 1 < ktiles)
+                load_tiles((kt + 1) * BK);
+            const T* as = As[buf];
+            const T* bs = Bs[buf];
+#pragma unroll
+            for(int k4 = 0; k4 < BK / 4; ++k4)
+            {
+                const int kr = k4 * 4 + lane / 16;
+                T         af[TM], bf[TN];
+#pragma unroll
+                for(int i = 0; i < TM; ++i)
+                    af[i] = as[kr * LDA_S + wm * WTM + i * 16 + (lane & 15)];
+#pragma unroll
+                for(int j = 0; j < TN; ++j)
+                    bf[j] = bs[kr * LDB_S + wn * WTN + j * 16 + (lane & 15)];
+#pragma unroll
+                for(int i = 0; i < TM; ++i)
+#pragma unroll
+                    for(int j = 0; j < TN; ++j)
+                        acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
+            }
+        }
+
+        // epilogue: lane holds column lane%16 and rows Mfma<T>::row(lane, r) of each tile
+#pragma unroll
+        for(int i = 0; i < TM; ++i)
+#pragma unroll
+            for(int j = 0; j < TN; ++j)
+            {
+                const int col = n0 + wn * WTN + j * 16 + (lane & 15);
+                if(col >= a.N)
+                    continue;
+#pragma unroll
+                for(int r = 0; r < 4; ++r)
+                {
+                    const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lane, r);
+                    if(row < a.M)
+                    {
+                        T& cref = Cp[row + int64_t(col) * a.ldc];
+                        T  v    = a.alpha * acc[i][j][r];
+                        if(a.beta != T(0))
+                            v = fma(a.beta, cref, v);
+                        cref = v;
+                    }
+                }
+            }
+        __syncthreads(); // LDS reuse across batch iterations
+    }
+}
+
+template <typename T, int BM, int BN, int BK, int WGM, int WGN>
+static rocblas_status launch_gemm_cfg(rocblas_handle h, GemmArgs<T> a)
+{
+    a.tiles_m         = (a.M + BM - 1) / BM;
+    a.tiles_n         = (a.N + BN - 1) / BN;
+    const int64_t nt  = int64_t(a.tiles_m) * a.tiles_n;
+    if(nt > int64_t(0x7fffffff))
+        return rocblas_status_invalid_size;
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WGM, WGN>), dim3(unsigned(nt), grid_batch(a.batch)),
+                       dim3(WAVE * WGM * WGN), 0, h->stream, a);
+    return launch_status();
+}
+
+// Pick the largest tile that still gives every CU work.
+template <typename T>
+rocblas_status launch_gemm(rocblas_handle h, const GemmArgs<T>& a)
+{
+    if(a.M <= 0 || a.N <= 0)
+        return rocblas_status_success;
+    const int64_t cus   = h->num_cus;
+    auto          tiles = [&](int bm, int bn) {
+        return int64_t((a.M + bm - 1) / bm) * ((a.N + bn - 1) / bn) * a.batch;
+    };
+    if(a.N <= 32)
+        return tiles(128, 32) >= cus ? launch_gemm_cfg<T, 128, 32, 16, 4, 1>(h, a)
+                                     : launch_gemm_cfg<T, 32, 32, 16, 2, 2>(h, a);
+    if(a.M <= 32)
+        return tiles(32, 128) >= cus ? launch_gemm_cfg<T, 32, 128, 16, 1, 4>(h, a)
+                                     : launch_gemm_cfg<T, 32, 32, 16, 2, 2>(h, a);
+    if(tiles(128, 128) >= cus)
+        return launch_gemm_cfg<T, 128, 128, 16, 2, 2>(h, a);
+    if(tiles(64, 64) >= cus)
+        return launch_gemm_cfg<T, 64, 64, 16, 2, 2>(h, a);
+    return launch_gemm_cfg<T, 32, 32, 16, 2, 2>(h, a);
+}
+
+
+template rocblas_status launch_gemm<float>(rocblas_handle, const GemmArgs<float>&);
+template rocblas_status launch_gemm<double>(rocblas_handle, const GemmArgs<double>&);
+
+} // namespace rbamd
