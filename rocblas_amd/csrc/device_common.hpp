@@ -48,6 +48,21 @@ struct Mfma<float>
     }
 };
 
+// Explicit global-address-space access: pointers read from LDS tables are
+// generic, and generic (flat) memory instructions also count against the LDS
+// wait counter, which would serialise LDS waits behind HBM loads.
+template <typename T>
+__device__ __forceinline__ T gload(const T* p)
+{
+    return *(const __attribute__((address_space(1))) T*)(p);
+}
+
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, T v)
+{
+    *(__attribute__((address_space(1))) T*)(p) = v;
+}
+
 inline rocblas_status launch_status()
 {
     hipError_t e = hipGetLastError();
@@ -69,7 +84,8 @@ enum SmallFlags : int
     SF_REV   = 4,  // canonical index i is real index K-1-i
     SF_UNIT  = 8,  // unit diagonal: the diagonal of A is never read
     SF_COLS  = 16, // the vectors are columns of B (side left)
-    SF_SCALE = 32  // multiply the right-hand side by alpha
+    SF_SCALE = 32, // multiply the right-hand side by alpha
+    SF_DPP   = 64  // register-resident triangles with DPP row broadcasts (KMAX <= 32)
 };
 
 template <typename T>
@@ -91,9 +107,8 @@ struct SmallArgs
     const T*          alpha_dev;
 };
 
-// Largest number of instance slots per wavefront for a given KMAX: keeps the
-// per-lane share of the slots' triangles at <= ~37 elements (all of them are
-// loaded before the first wait) and the LDS region near 17-25 KiB.
+// Largest number of instance slots per wavefront of the LDS variant for a
+// given KMAX: keeps the triangles' LDS region near 17-19 KiB.
 constexpr int small_gmax(int kmax)
 {
     return kmax <= 8 ? 64 : kmax <= 16 ? 16 : kmax <= 32 ? 4 : kmax <= 48 ? 2 : 1;

@@ -82,23 +82,32 @@ struct SmallPlan
     int    G;
     int    slot_stride;
     int    bs_stride;
+    size_t lds_elems;
     size_t lds_bytes;
 };
 
-// vectors per slot: the next power of two >= nvec (<= 64); slots per
-// wavefront G = 64 / VP, capped by small_gmax(KMAX) by widening the slots.
+// Two kernel variants (small_trsm.hpp):
+//   DPP  (KMAX <= 32 and more than 8 vectors per instance): slots of
+//        VP >= 16 vectors, triangles in registers; LDS only stages side-L
+//        right-hand sides.
+//   LDS  (otherwise): VP = next power of two >= nvec, G = 64 / VP slots per
+//        wavefront capped by small_gmax(KMAX) (by widening the slots);
+//        triangles in LDS.
 template <typename T>
-static SmallPlan plan_small(int K, int nvec, bool cols)
+static SmallPlan plan_small(int K, int nvec, bool cols, bool& dpp)
 {
     SmallPlan s;
     s.kmax        = K <= 8 ? 8 : K <= 16 ? 16 : K <= 32 ? 32 : K <= 48 ? 48 : 64;
     s.slot_stride = slot_elems(s.kmax);
     s.bs_stride   = s.kmax + 1;
+    dpp           = s.kmax <= 32 && nvec > 8;
     int vp        = 1;
     while(vp < std::min(nvec, WAVE))
         vp <<= 1;
+    if(dpp)
+        vp = std::max(vp, 16);
     int G = WAVE / vp;
-    while(G > small_gmax(s.kmax))
+    while(!dpp && G > small_gmax(s.kmax))
     {
         G >>= 1;
         vp <<= 1;
@@ -107,8 +116,10 @@ static SmallPlan plan_small(int K, int nvec, bool cols)
     s.vp_log2 = 0;
     while((1 << s.vp_log2) < vp)
         ++s.vp_log2;
-    const size_t staging = cols ? size_t(WAVE) * s.bs_stride : 0;
-    s.lds_bytes          = std::max(size_t(G) * s.slot_stride, staging) * sizeof(T);
+    // staging: side L for the DPP variant, both sides for the LDS variant
+    const size_t staging = (cols || !dpp) ? size_t(WAVE) * s.bs_stride : 0;
+    s.lds_elems          = (dpp ? 0 : size_t(G) * s.slot_stride) + staging;
+    s.lds_bytes          = s.lds_elems * sizeof(T);
     return s;
 }
 
@@ -117,7 +128,10 @@ static rocblas_status launch_small(rocblas_handle h, const SmallArgs<T>& args_in
 {
     SmallArgs<T>    a    = args_in;
     const bool      cols = a.flags & SF_COLS;
-    const SmallPlan s    = plan_small<T>(a.K, a.nvec, cols);
+    bool            dpp  = false;
+    const SmallPlan s    = plan_small<T>(a.K, a.nvec, cols, dpp);
+    if(dpp)
+        a.flags |= SF_DPP;
     const int       VP   = 1 << s.vp_log2;
     a.vp_log2            = s.vp_log2;
     a.chunks             = (a.nvec + VP - 1) / VP;
