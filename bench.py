@@ -26,6 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per side-left launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE
+# (copy-calibrated); see the profiles/ summary named below.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "small_trsm_strided_batched_m32n16.traffic.json")
 CASES = [(s, u, t) for s in "LR" for u in "LU" for t in "NT"]
 BATCH, M, N = 20000, 32, 16
 
@@ -106,6 +109,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain device copy of 248,320,000 bytes (the side-L "
+                         "launch's algorithmic bytes), so a rocprofv3 --pmc run of this "
+                         "command reads the counters of both")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the triangular-solve path is single-GPU here "
@@ -162,6 +169,18 @@ def main():
     achieved = BATCH * algo_bytes("L") / left_avg_s / 1e9
 
     eta = check_accuracy(torch, rb, h, data, Bsrc)
+    if args.copy_calibration:
+        # read 124,160,000 B + write 124,160,000 B
+        src = torch.ones(BATCH * algo_bytes("L") // 16, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
     solves = len(CASES) * BATCH
     flops = sum(BATCH * algo_flops(c[0]) for c in CASES)
     out = {
@@ -186,9 +205,10 @@ def main():
                        "p99": float(np.percentile(all_us, 99)),
                        "left_avg": float(left_us.mean()), "right_avg": float(right_us.mean())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "small_trsm_kernel<double,32> (side L)",
-                     "algorithmic_bytes_per_launch": BATCH * algo_bytes("L")},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "small_trsm_dpp_kernel<double,32,*,*,cols> (the 4 side-left cases)",
+                     "algorithmic_bytes_per_launch": BATCH * algo_bytes("L"),
+                     "avg_launch_us": float(left_us.mean())},
     }
     if not args.no_cpu:
         import oracle
