@@ -237,6 +237,79 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_dtrsm_strided_batched(rocblas_handle  
                                                                 rocblas_stride    stride_b,
                                                                 rocblas_int       batch_count);
 
+/* ---- triangular inverse --------------------------------------------------- */
+/* invA := inverse of the uplo triangle of A (n x n); the opposite triangle of
+ * invA is cleared.  invA may be A itself.  Checks follow
+ * library/src/blas3/rocblas_trtri.hpp:31-62. */
+
+/* rocblas-functions.h:19081 */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_strtri(rocblas_handle   handle,
+                                                 rocblas_fill     uplo,
+                                                 rocblas_diagonal diag,
+                                                 rocblas_int      n,
+                                                 const float*     A,
+                                                 rocblas_int      lda,
+                                                 float*           invA,
+                                                 rocblas_int      ldinvA);
+
+/* rocblas-functions.h:19090 */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_dtrtri(rocblas_handle   handle,
+                                                 rocblas_fill     uplo,
+                                                 rocblas_diagonal diag,
+                                                 rocblas_int      n,
+                                                 const double*    A,
+                                                 rocblas_int      lda,
+                                                 double*          invA,
+                                                 rocblas_int      ldinvA);
+
+/* rocblas-functions.h:19159 */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_strtri_batched(rocblas_handle     handle,
+                                                         rocblas_fill       uplo,
+                                                         rocblas_diagonal   diag,
+                                                         rocblas_int        n,
+                                                         const float* const A[],
+                                                         rocblas_int        lda,
+                                                         float* const       invA[],
+                                                         rocblas_int        ldinvA,
+                                                         rocblas_int        batch_count);
+
+/* rocblas-functions.h:19169 */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_dtrtri_batched(rocblas_handle      handle,
+                                                         rocblas_fill        uplo,
+                                                         rocblas_diagonal    diag,
+                                                         rocblas_int         n,
+                                                         const double* const A[],
+                                                         rocblas_int         lda,
+                                                         double* const       invA[],
+                                                         rocblas_int         ldinvA,
+                                                         rocblas_int         batch_count);
+
+/* rocblas-functions.h:19249 */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_strtri_strided_batched(rocblas_handle   handle,
+                                                                 rocblas_fill     uplo,
+                                                                 rocblas_diagonal diag,
+                                                                 rocblas_int      n,
+                                                                 const float*     A,
+                                                                 rocblas_int      lda,
+                                                                 rocblas_stride   stride_a,
+                                                                 float*           invA,
+                                                                 rocblas_int      ldinvA,
+                                                                 rocblas_stride   stride_invA,
+                                                                 rocblas_int      batch_count);
+
+/* rocblas-functions.h:19261 */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_dtrtri_strided_batched(rocblas_handle   handle,
+                                                                 rocblas_fill     uplo,
+                                                                 rocblas_diagonal diag,
+                                                                 rocblas_int      n,
+                                                                 const double*    A,
+                                                                 rocblas_int      lda,
+                                                                 rocblas_stride   stride_a,
+                                                                 double*          invA,
+                                                                 rocblas_int      ldinvA,
+                                                                 rocblas_stride   stride_invA,
+                                                                 rocblas_int      batch_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,10 @@ def lib():
         L.oracle_splitmix_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t,
                                               ctypes.c_double, ctypes.c_double, dp]
         L.oracle_splitmix_uniform.restype = None
+        L.oracle_trtri_d.argtypes = [i, i, i, dp, i]
+        L.oracle_trtri_d.restype = None
+        L.oracle_trtri_s.argtypes = [i, i, i, fp, i]
+        L.oracle_trtri_s.restype = None
         _lib = L
     return _lib
 
@@ -108,6 +112,15 @@ def trsm(side, uplo, trans, diag, m, n, alpha, A, B):
         lib().oracle_trsm_s(s, u, t, d, m, n, float(alpha), _ptr(A), A.shape[0], _ptr(B),
                             B.shape[0])
     return B
+
+
+def trtri(uplo, diag, n, A):
+    """In-place inverse of the ``uplo`` triangle of A (LAPACK xTRTI2 loop nest);
+    the other triangle is left untouched, as in LAPACK."""
+    _check_colmajor(A, "A")
+    fn = lib().oracle_trtri_d if A.dtype == np.float64 else lib().oracle_trtri_s
+    fn(FILL[uplo], DIAG[diag], n, _ptr(A), A.shape[0])
+    return A
 
 
 def trmm(side, uplo, trans, diag, m, n, alpha, A, B):

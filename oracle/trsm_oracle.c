@@ -32,6 +32,9 @@
  *     u_i = mix(key + (i+1)*0x9E3779B97F4A7C15) >> 11 * 2^-53.
  *   - oracle_trsm_*_batched_omp: the same loop nest run over a strided batch,
  *     OpenMP-parallel over instances: the CPU baseline.
+ *   - oracle_trtri_{d,s}: in-place triangular inverse, the unblocked LAPACK
+ *     xTRTI2 algorithm that the reference's host comparator lapack_xtrtri
+ *     (clients/include/lapack_utilities.hpp:625-708) uses on its blocks.
  *
  * Enum values are rocBLAS's (library/include/internal/rocblas-types.h:125-157),
  * which coincide with CBLAS's: side 141/142, fill 121/122, op 111/112/113,
@@ -560,3 +563,79 @@ void oracle_splitmix_uniform(uint64_t key, uint64_t offset, size_t count, double
         out[i]     = lo + (hi - lo) * u;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Triangular inverse, in place: the unblocked LAPACK xTRTI2 algorithm,     */
+/* which the reference's host comparator lapack_xtrtri                      */
+/* (clients/include/lapack_utilities.hpp:625-708) applies per 64-wide       */
+/* block.  Column j of the inverse is the already-inverted triangle times  */
+/* column j (an xTRMV), scaled by -1/A(j,j).                                */
+/* ------------------------------------------------------------------------ */
+#define DEFINE_TRTRI(NAME, T)                                                        \
+    void NAME(int uplo, int diag, int n, T* A, int lda)                              \
+    {                                                                                \
+        const int nounit = (diag == DIAG_N);                                         \
+        int       i, j, k;                                                           \
+        if(uplo == FILL_U)                                                           \
+        {                                                                            \
+            for(j = 0; j < n; j++)                                                   \
+            {                                                                        \
+                T ajj;                                                               \
+                if(nounit)                                                           \
+                {                                                                    \
+                    A[IDX(j, j, lda)] = (T)1 / A[IDX(j, j, lda)];                    \
+                    ajj               = -A[IDX(j, j, lda)];                          \
+                }                                                                    \
+                else                                                                 \
+                    ajj = (T)-1;                                                     \
+                /* x := U * x, U = inverted leading j x j block, x = A(0:j, j) */   \
+                for(k = 0; k < j; k++)                                               \
+                {                                                                    \
+                    T temp = A[IDX(k, j, lda)];                                      \
+                    if(temp != (T)0)                                                 \
+                    {                                                                \
+                        for(i = 0; i < k; i++)                                       \
+                            A[IDX(i, j, lda)] += temp * A[IDX(i, k, lda)];           \
+                        if(nounit)                                                   \
+                            A[IDX(k, j, lda)] = temp * A[IDX(k, k, lda)];            \
+                    }                                                                \
+                }                                                                    \
+                for(i = 0; i < j; i++)                                               \
+                    A[IDX(i, j, lda)] *= ajj;                                        \
+            }                                                                        \
+        }                                                                            \
+        else                                                                         \
+        {                                                                            \
+            for(j = n - 1; j >= 0; j--)                                              \
+            {                                                                        \
+                T ajj;                                                               \
+                if(nounit)                                                           \
+                {                                                                    \
+                    A[IDX(j, j, lda)] = (T)1 / A[IDX(j, j, lda)];                    \
+                    ajj               = -A[IDX(j, j, lda)];                          \
+                }                                                                    \
+                else                                                                 \
+                    ajj = (T)-1;                                                     \
+                if(j < n - 1)                                                        \
+                {                                                                    \
+                    /* x := L * x, L = inverted trailing block, x = A(j+1:n, j) */  \
+                    for(k = n - 1; k > j; k--)                                       \
+                    {                                                                \
+                        T temp = A[IDX(k, j, lda)];                                  \
+                        if(temp != (T)0)                                             \
+                        {                                                            \
+                            for(i = n - 1; i > k; i--)                               \
+                                A[IDX(i, j, lda)] += temp * A[IDX(i, k, lda)];       \
+                            if(nounit)                                               \
+                                A[IDX(k, j, lda)] = temp * A[IDX(k, k, lda)];        \
+                        }                                                            \
+                    }                                                                \
+                    for(i = j + 1; i < n; i++)                                       \
+                        A[IDX(i, j, lda)] *= ajj;                                    \
+                }                                                                    \
+            }                                                                        \
+        }                                                                            \
+    }
+
+DEFINE_TRTRI(oracle_trtri_d, double)
+DEFINE_TRTRI(oracle_trtri_s, float)

@@ -86,6 +86,10 @@ _i64 = ctypes.c_int64
 # (name, restype, argtypes) for every symbol declared in include/rocblas_amd.h
 _TRSM_PLAIN = [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32]
 _TRSM_BATCHED = [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _i32]
+_TRTRI_PLAIN = [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32]
+_TRTRI_BATCHED = [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32]
+_TRTRI_STRIDED = [_vp, _i32, _i32, _i32, _vp, _i32, _i64, _vp, _i32, _i64, _i32]
+_TRSM_VBATCHED = [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32]
 _TRSM_STRIDED = [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _i64, _vp, _i32, _i64,
                  _i32]
 SYMBOLS = [
@@ -110,6 +114,22 @@ SYMBOLS = [
     ("rocblas_dtrsm_batched", _i32, _TRSM_BATCHED),
     ("rocblas_strsm_strided_batched", _i32, _TRSM_STRIDED),
     ("rocblas_dtrsm_strided_batched", _i32, _TRSM_STRIDED),
+    ("rocblas_strtri", _i32, _TRTRI_PLAIN),
+    ("rocblas_dtrtri", _i32, _TRTRI_PLAIN),
+    ("rocblas_strtri_batched", _i32, _TRTRI_BATCHED),
+    ("rocblas_dtrtri_batched", _i32, _TRTRI_BATCHED),
+    ("rocblas_strtri_strided_batched", _i32, _TRTRI_STRIDED),
+    ("rocblas_dtrtri_strided_batched", _i32, _TRTRI_STRIDED),
+    # include/rocblas_amd_ext.h
+    ("rocblas_amd_strsm_vbatched", _i32, _TRSM_VBATCHED),
+    ("rocblas_amd_dtrsm_vbatched", _i32, _TRSM_VBATCHED),
+    ("rocblas_amd_vbatched_set_tuning", _i32, [_i32, _i32]),
+    ("rocblas_amd_dsynth_tri", _i32, [_vp, _i32, _i32, _i32, ctypes.c_uint64, ctypes.c_double,
+                                      ctypes.c_double, _vp, _i32]),
+    ("rocblas_amd_ssynth_tri", _i32, [_vp, _i32, _i32, _i32, ctypes.c_uint64, ctypes.c_double,
+                                      ctypes.c_double, _vp, _i32]),
+    ("rocblas_amd_dsynth_rhs", _i32, [_vp, _i32, _i32, ctypes.c_uint64, _i32, _vp, _i32]),
+    ("rocblas_amd_ssynth_rhs", _i32, [_vp, _i32, _i32, ctypes.c_uint64, _i32, _vp, _i32]),
 ]
 
 
@@ -174,12 +194,6 @@ class Handle:
         return lib().rocblas_set_pointer_mode(self.ptr, int(mode))
 
 
-def _scalar(value, dtype_char):
-    if dtype_char == "d":
-        return ctypes.byref(ctypes.c_double(value)), ctypes.c_double(value)
-    return ctypes.byref(ctypes.c_float(value)), ctypes.c_float(value)
-
-
 def trsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, precision="d",
          alpha_ptr=None):
     """rocblas_{s,d}trsm on raw device pointers (ints).  ``alpha`` is a host
@@ -226,6 +240,57 @@ def trsm_batched(handle, side, uplo, trans, diag, m, n, alpha, A_array, lda, B_a
             ldb, batch)
     del keep
     return rocblas_status(st)
+
+
+def trtri(handle, uplo, diag, n, A, lda, invA, ldinvA, precision="d"):
+    """rocblas_{s,d}trtri on raw device pointers; returns the status."""
+    fn = lib().rocblas_dtrtri if precision == "d" else lib().rocblas_strtri
+    return rocblas_status(fn(_h(handle), int(FILL.get(uplo, uplo)), int(DIAG.get(diag, diag)), n,
+                             A, lda, invA, ldinvA))
+
+
+def trtri_strided_batched(handle, uplo, diag, n, A, lda, stride_a, invA, ldinvA, stride_invA,
+                          batch, precision="d"):
+    fn = (lib().rocblas_dtrtri_strided_batched if precision == "d"
+          else lib().rocblas_strtri_strided_batched)
+    return rocblas_status(fn(_h(handle), int(FILL.get(uplo, uplo)), int(DIAG.get(diag, diag)), n,
+                             A, lda, stride_a, invA, ldinvA, stride_invA, batch))
+
+
+def trtri_batched(handle, uplo, diag, n, A_array, lda, invA_array, ldinvA, batch, precision="d"):
+    fn = lib().rocblas_dtrtri_batched if precision == "d" else lib().rocblas_strtri_batched
+    return rocblas_status(fn(_h(handle), int(FILL.get(uplo, uplo)), int(DIAG.get(diag, diag)), n,
+                             A_array, lda, invA_array, ldinvA, batch))
+
+
+def trsm_vbatched(handle, side, uplo, trans, diag, m_arr, n_arr, alpha, A_array, lda_arr, B_array,
+                  ldb_arr, batch, precision="d", alpha_ptr=None):
+    """rocblas_amd_{s,d}trsm_vbatched: every size/pointer argument is a device pointer."""
+    fn = lib().rocblas_amd_dtrsm_vbatched if precision == "d" else lib().rocblas_amd_strsm_vbatched
+    a = alpha_ptr
+    keep = None
+    if a is None and alpha is not None:
+        keep = ctypes.c_double(alpha) if precision == "d" else ctypes.c_float(alpha)
+        a = ctypes.addressof(keep)
+    st = fn(_h(handle), int(SIDE.get(side, side)), int(FILL.get(uplo, uplo)),
+            int(OP.get(trans, trans)), int(DIAG.get(diag, diag)), m_arr, n_arr, a, A_array,
+            lda_arr, B_array, ldb_arr, batch)
+    del keep
+    return rocblas_status(st)
+
+
+def synth_tri(handle, uplo, dist, k, seed, A, lda, scale=1.0, poison=0.0, precision="d"):
+    """Device-side synthetic triangle (dist "uniform" or "normal") into A."""
+    fn = lib().rocblas_amd_dsynth_tri if precision == "d" else lib().rocblas_amd_ssynth_tri
+    return rocblas_status(fn(_h(handle), int(FILL.get(uplo, uplo)),
+                             {"uniform": 0, "normal": 1}.get(dist, dist), k, seed, scale, poison,
+                             A, lda))
+
+
+def synth_rhs(handle, m, n, seed, B, ldb, tag=2, precision="d"):
+    """Device-side U[-1,1] right-hand side into B."""
+    fn = lib().rocblas_amd_dsynth_rhs if precision == "d" else lib().rocblas_amd_ssynth_rhs
+    return rocblas_status(fn(_h(handle), m, n, seed, tag, B, ldb))
 
 
 def _h(handle):

@@ -138,4 +138,11 @@ rocblas_status zero_matrix_launch(rocblas_handle h, int m, int n, BatchPtr<T> B,
 template <typename T>
 size_t trsm_workspace_bytes(const TrsmProblem<T>& p);
 
+template <typename T>
+rocblas_status identity_launch(rocblas_handle h, int n, BatchPtr<T> X, int ldx, int batch);
+
+template <typename T>
+rocblas_status copy_launch(rocblas_handle h, int m, int n, BatchPtr<const T> S, int lds,
+                           BatchPtr<T> D, int ldd, int batch);
+
 } // namespace rbamd

@@ -73,6 +73,74 @@ __global__ void __launch_bounds__(256) scale_kernel(int m, int n, BatchPtr<T> B,
 }
 
 // ---------------------------------------------------------------------------
+// X := I (n x n, leading dimension ldx) over a batch, and a plain matrix copy;
+// used by trtri, which solves op(A) X = I in place on invA.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) identity_kernel(int n, BatchPtr<T> X, int ldx, int batch)
+{
+    const int64_t total = int64_t(n) * n;
+    for(int64_t b = blockIdx.y; b < batch; b += gridDim.y)
+    {
+        T* p = X.get(b);
+        for(int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total;
+            e += int64_t(gridDim.x) * 256)
+        {
+            int64_t j = e / n, i = e - j * n;
+            p[i + j * ldx] = i == j ? T(1) : T(0);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) copy_kernel(int m, int n, BatchPtr<const T> S, int lds,
+                                                    BatchPtr<T> D, int ldd, int batch)
+{
+    const int64_t total = int64_t(m) * n;
+    for(int64_t b = blockIdx.y; b < batch; b += gridDim.y)
+    {
+        const T* s = S.get(b);
+        T*       d = D.get(b);
+        for(int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total;
+            e += int64_t(gridDim.x) * 256)
+        {
+            int64_t j = e / m, i = e - j * m;
+            d[i + j * ldd] = s[i + j * lds];
+        }
+    }
+}
+
+template <typename T>
+rocblas_status identity_launch(rocblas_handle h, int n, BatchPtr<T> X, int ldx, int batch)
+{
+    const int64_t  total = int64_t(n) * n;
+    const unsigned bx    = unsigned(std::min<int64_t>((total + 255) / 256, 4096));
+    DeviceGuard    g(h->device);
+    hipLaunchKernelGGL(identity_kernel<T>, dim3(bx, grid_batch(batch)), dim3(256), 0, h->stream,
+                       n, X, ldx, batch);
+    return launch_status();
+}
+
+template <typename T>
+rocblas_status copy_launch(rocblas_handle h, int m, int n, BatchPtr<const T> S, int lds,
+                           BatchPtr<T> D, int ldd, int batch)
+{
+    const int64_t  total = int64_t(m) * n;
+    const unsigned bx    = unsigned(std::min<int64_t>((total + 255) / 256, 4096));
+    DeviceGuard    g(h->device);
+    hipLaunchKernelGGL(copy_kernel<T>, dim3(bx, grid_batch(batch)), dim3(256), 0, h->stream, m,
+                       n, S, lds, D, ldd, batch);
+    return launch_status();
+}
+
+template rocblas_status identity_launch<float>(rocblas_handle, int, BatchPtr<float>, int, int);
+template rocblas_status identity_launch<double>(rocblas_handle, int, BatchPtr<double>, int, int);
+template rocblas_status copy_launch<float>(rocblas_handle, int, int, BatchPtr<const float>, int,
+                                           BatchPtr<float>, int, int);
+template rocblas_status copy_launch<double>(rocblas_handle, int, int, BatchPtr<const double>, int,
+                                            BatchPtr<double>, int, int);
+
+// ---------------------------------------------------------------------------
 // Small-solve launch plan
 // ---------------------------------------------------------------------------
 struct SmallPlan

@@ -13,11 +13,16 @@ import rocblas_amd as rb
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "rocblas_amd.h")
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if f.endswith(".h")]
 
 
 def declared_symbols():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"ROCBLAS_AMD_EXPORT[^;(]*?\b(rocblas_\w+)\s*\(", text)))
+    names = set()
+    for hdr in HEADERS:
+        text = open(hdr).read()
+        names |= set(re.findall(r"ROCBLAS_AMD_EXPORT[^;(]*?\b(rocblas_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_the_trsm_path():
@@ -43,7 +48,8 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_header_compiles_as_c():
-    src = '#include "rocblas_amd.h"\nint main(void){ return (int)rocblas_status_success; }\n'
+    src = ('#include "rocblas_amd.h"\n#include "rocblas_amd_ext.h"\n'
+           'int main(void){ return (int)rocblas_status_success; }\n')
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.dirname(HEADER),
                         "-x", "c", "-", "-o", "/dev/null"], input=src, text=True,
                        capture_output=True)

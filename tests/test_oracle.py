@@ -275,3 +275,37 @@ def test_synth_matrices_shape_and_distribution():
     assert Ab[2, 4, 4] == (1 + u[4, 4]) / k
     Bm = synth.rhs_uniform(8, 3, 1, ld=10)
     assert Bm.shape == (10, 3) and np.all(Bm[8:] == 0) and np.all(np.abs(Bm[:8]) <= 1)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_trtri_against_numpy_and_exact(uplo, diag):
+    """xTRTI2 restatement: against numpy's inverse of the triangle, an exactly
+    invertible integer case (I + N with N nilpotent bidiagonal: the inverse is a
+    triangle of +-1 powers), and the untouched opposite triangle."""
+    rng = np.random.default_rng(31)
+    for n in (1, 2, 7, 40, 65):
+        A = oracle.diag_dominant_triangular(n, uplo, rng, lda=n + 2)
+        if diag == "U":
+            oracle.make_unit_diagonal(A, uplo, n)
+        T = op_dense(A, uplo, "N", diag, n)
+        want = np.linalg.inv(T)
+        got = A.copy(order="F")
+        oracle.trtri(uplo, diag, n, got)
+        inv = op_dense(got, uplo, "N", "N" if diag == "N" else "U", n)
+        assert np.abs(inv - want).max() <= 1e-13 * max(1.0, np.abs(want).max())
+        # the opposite triangle is not touched (LAPACK semantics)
+        off = np.triu(np.ones((n, n), bool), 1) if uplo == "L" else np.tril(np.ones((n, n), bool), -1)
+        assert np.all(np.isnan(got[:n, :n][off]))
+    n = 30
+    A = np.zeros((n, n), order="F")
+    np.fill_diagonal(A, 1.0)
+    for i in range(n - 1):
+        if uplo == "L":
+            A[i + 1, i] = -1.0
+        else:
+            A[i, i + 1] = -1.0
+    got = A.copy(order="F")
+    oracle.trtri(uplo, "N", n, got)
+    want = np.tril(np.ones((n, n))) if uplo == "L" else np.triu(np.ones((n, n)))
+    assert np.array_equal(got, want)

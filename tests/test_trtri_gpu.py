@@ -1,0 +1,170 @@
+"""GPU parity tests of rocblas_{s,d}trtri{,_batched,_strided_batched} through the C ABI.
+
+Mirrors clients/include/blas3/testing_trtri.hpp: the bad-argument table, the
+comparison against the host xTRTI2 (here the oracle's restatement) at the
+reference's tolerance eps * max(10000, N) for double (clients/include/unit.hpp:
+631-638), plus exact integer inverses, the cleared opposite triangle, in-place
+calls and batched forms.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import rocblas_amd as rb
+from tests.gpu_util import GpuHandle, dev, host, ptr_array, say
+from tests.helpers import op_dense
+
+pytestmark = pytest.mark.gpu
+S = rb.rocblas_status
+EPS64 = np.finfo(np.float64).eps
+EPS32 = np.finfo(np.float32).eps
+
+
+@pytest.fixture(scope="module")
+def h():
+    assert torch.cuda.is_available()
+    oracle.build()
+    hd = GpuHandle()
+    yield hd
+    hd.close()
+
+
+def tri_of(X, uplo, n, unit):
+    T = np.triu(X[:n, :n]) if uplo == "U" else np.tril(X[:n, :n])
+    if unit:
+        np.fill_diagonal(T, 1.0)
+    return T
+
+
+def test_bad_args(h):
+    """clients/include/blas3/testing_trtri.hpp:42-90"""
+    say("trtri bad args")
+    N = lda = 100
+    dA = torch.zeros(N * lda, dtype=torch.float64, device="cuda")
+    dI = torch.zeros(N * lda, dtype=torch.float64, device="cuda")
+    L = rb.lib()
+    A, I = dA.data_ptr(), dI.data_ptr()
+    assert L.rocblas_dtrtri(h.ptr, 123, 131, N, A, lda, I, lda) == S.invalid_value
+    assert L.rocblas_dtrtri(h.ptr, 121, 143, N, A, lda, I, lda) == S.invalid_value
+    assert L.rocblas_dtrtri(h.ptr, 121, 131, -1, A, lda, I, lda) == S.invalid_size
+    assert L.rocblas_dtrtri(h.ptr, 121, 131, N, A, lda - 1, I, lda) == S.invalid_size
+    assert L.rocblas_dtrtri(h.ptr, 121, 131, N, A, lda, I, lda - 1) == S.invalid_size
+    assert L.rocblas_dtrtri(None, 121, 131, N, A, lda, I, lda) == S.invalid_handle
+    assert L.rocblas_dtrtri(h.ptr, 121, 131, N, None, lda, I, lda) == S.invalid_pointer
+    assert L.rocblas_dtrtri(h.ptr, 121, 131, N, A, lda, None, lda) == S.invalid_pointer
+    assert L.rocblas_dtrtri(h.ptr, 121, 131, 0, None, lda, None, lda) == S.success
+    assert L.rocblas_dtrtri_strided_batched(h.ptr, 121, 131, N, A, lda, 0, I, lda, 0,
+                                            -1) == S.invalid_size
+    assert L.rocblas_dtrtri_batched(h.ptr, 121, 131, N, None, lda, None, lda, 0) == S.success
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_against_oracle(h, uplo, diag):
+    rng = np.random.default_rng(41)
+    for n in (1, 5, 32, 63, 64, 65, 130, 333):
+        say("trtri", uplo, diag, n)
+        A = oracle.diag_dominant_triangular(n, uplo, rng, lda=n + 3)  # NaN elsewhere
+        if diag == "U":
+            oracle.make_unit_diagonal(A, uplo, n)
+        ref = A.copy(order="F")
+        oracle.trtri(uplo, diag, n, ref)
+        dA = dev(A)
+        dI = torch.full(((n + 1) * n,), 7.0, dtype=torch.float64, device="cuda")
+        st = rb.trtri(h, uplo, diag, n, dA.data_ptr(), n + 3, dI.data_ptr(), n + 1)
+        assert st == S.success
+        torch.cuda.synchronize()
+        got = host(dI, (n + 1, n))
+        R = tri_of(ref, uplo, n, diag == "U")
+        G = got[:n, :n]
+        tol = EPS64 * max(10000.0, n)
+        assert np.abs(G - R).max() / max(1.0, np.abs(R).max()) <= tol
+        # opposite triangle cleared, padding rows untouched, input unchanged
+        off = np.triu(np.ones((n, n), bool), 1) if uplo == "L" else np.tril(np.ones((n, n), bool), -1)
+        assert np.all(G[off] == 0.0)
+        assert np.all(got[n:, :] == 7.0)
+        assert np.array_equal(host(dA, A.shape), A, equal_nan=True)
+        # invA * A = I
+        T = op_dense(A, uplo, "N", diag, n)
+        assert np.abs(G @ T - np.eye(n)).max() <= 1e-12
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_exact_integer_inverse(h, uplo):
+    """I minus the unit sub/super-diagonal has the all-ones triangle as inverse:
+    every value on the way is a small integer, so the result is exact."""
+    n = 200
+    A = np.zeros((n, n), order="F")
+    np.fill_diagonal(A, 1.0)
+    for i in range(n - 1):
+        if uplo == "L":
+            A[i + 1, i] = -1.0
+        else:
+            A[i, i + 1] = -1.0
+    dA = dev(A)
+    dI = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    assert rb.trtri(h, uplo, "N", n, dA.data_ptr(), n, dI.data_ptr(), n) == S.success
+    torch.cuda.synchronize()
+    want = np.tril(np.ones((n, n))) if uplo == "L" else np.triu(np.ones((n, n)))
+    assert np.array_equal(host(dI, (n, n)), want)
+
+
+def test_in_place_and_batched_forms(h):
+    rng = np.random.default_rng(43)
+    n, batch = 70, 5
+    for uplo in "UL":
+        say("trtri batched", uplo)
+        As = [oracle.diag_dominant_triangular(n, uplo, rng, poison=0.0) for _ in range(batch)]
+        singles = []
+        for A in As:
+            dA = dev(A)
+            dI = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+            assert rb.trtri(h, uplo, "N", n, dA.data_ptr(), n, dI.data_ptr(), n) == S.success
+            singles.append(dI)
+        torch.cuda.synchronize()
+        # in place: invA == A
+        dA = dev(As[0])
+        assert rb.trtri(h, uplo, "N", n, dA.data_ptr(), n, dA.data_ptr(), n) == S.success
+        torch.cuda.synchronize()
+        assert torch.equal(dA, singles[0])
+        # strided batched
+        stride = n * n + 11
+        flatA = np.zeros(stride * batch)
+        for b, A in enumerate(As):
+            flatA[b * stride:b * stride + n * n] = A.ravel(order="F")
+        dAs = torch.from_numpy(flatA).cuda()
+        dIs = torch.zeros(stride * batch, dtype=torch.float64, device="cuda")
+        assert rb.trtri_strided_batched(h, uplo, "N", n, dAs.data_ptr(), n, stride, dIs.data_ptr(),
+                                        n, stride, batch) == S.success
+        # pointer-array batched
+        dAl = [dev(A) for A in As]
+        dIl = [torch.zeros(n * n, dtype=torch.float64, device="cuda") for _ in As]
+        pa, pi = ptr_array(dAl), ptr_array(dIl)
+        assert rb.trtri_batched(h, uplo, "N", n, pa.data_ptr(), n, pi.data_ptr(), n,
+                                batch) == S.success
+        torch.cuda.synchronize()
+        for b in range(batch):
+            assert torch.equal(dIs[b * stride:b * stride + n * n], singles[b])
+            assert torch.equal(dIl[b], singles[b])
+
+
+def test_float32(h):
+    rng = np.random.default_rng(47)
+    for uplo in "UL":
+        n = 90
+        A = oracle.diag_dominant_triangular(n, uplo, rng, poison=0.0).astype(np.float32)
+        A = np.asfortranarray(A)
+        ref = A.copy(order="F")
+        oracle.trtri(uplo, "N", n, ref)
+        dA = dev(A)
+        dI = torch.zeros(n * n, dtype=torch.float32, device="cuda")
+        assert rb.trtri(h, uplo, "N", n, dA.data_ptr(), n, dI.data_ptr(), n,
+                        precision="s") == S.success
+        torch.cuda.synchronize()
+        G = host(dI, (n, n))
+        R = tri_of(ref, uplo, n, False)
+        assert np.abs(G - R).max() / np.abs(R).max() <= EPS32 * max(1000.0, n)
