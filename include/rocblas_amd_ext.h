@@ -66,6 +66,50 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_dtrsm_vbatched(rocblas_handle     
                                                              const rocblas_int*  ldb,
                                                              rocblas_int         batch_count);
 
+/* Low-precision form of the trailing updates of rocblas_amd_dtrsm_mixed. */
+typedef enum rocblas_amd_precision_form_
+{
+    /* FP32 copy of A, FP32 MFMA products accumulated in FP32 */
+    rocblas_amd_form_fp32 = 0
+} rocblas_amd_precision_form;
+
+/* Outcome of rocblas_amd_dtrsm_mixed (host struct, optional). */
+typedef struct rocblas_amd_refine_info_
+{
+    rocblas_int iterations;     /* refinement steps applied after the first solve */
+    double      backward_error; /* |alpha B - op(A) X|_1 / (|op(A)|_1 |X|_1 + |alpha| |B|_1)
+                                   of the returned X; -1 when tol <= 0 (not measured) */
+    rocblas_int converged;      /* 1 when backward_error <= tol (or tol <= 0) */
+} rocblas_amd_refine_info;
+
+/* FP64 triangular solve, same problem and argument rules as rocblas_dtrsm,
+ * computed with low-precision trailing updates and FP64 iterative refinement:
+ * FP64 inverses of the block x block diagonal blocks of A (trtri), trailing
+ * updates on FP32 MFMA, FP64 residual alpha B - op(A) X, and corrections
+ * until the normwise backward error is <= tol or max_refine steps are taken.
+ * block = 0 picks the tuned block size.  tol <= 0 runs exactly max_refine
+ * steps without measuring (no host synchronisation); tol > 0 synchronises
+ * the stream once per step to test convergence.  Workspace (library-managed
+ * handles grow on demand): 2 m n + k^2/2 + nblocks block^2 doubles.
+ * No counterpart in the reference. */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_dtrsm_mixed(rocblas_handle             handle,
+                                                          rocblas_side               side,
+                                                          rocblas_fill               uplo,
+                                                          rocblas_operation          transA,
+                                                          rocblas_diagonal           diag,
+                                                          rocblas_int                m,
+                                                          rocblas_int                n,
+                                                          const double*              alpha,
+                                                          const double*              A,
+                                                          rocblas_int                lda,
+                                                          double*                    B,
+                                                          rocblas_int                ldb,
+                                                          rocblas_amd_precision_form form,
+                                                          rocblas_int                block,
+                                                          rocblas_int                max_refine,
+                                                          double                     tol,
+                                                          rocblas_amd_refine_info*   info);
+
 /* Seeded synthetic inputs generated in place on the handle's stream, with the
  * counter-based SplitMix64 streams of rocblas_amd/synth.py: the k x k
  * triangle `uplo` of A (every entry times `scale`; the other triangle and rows

@@ -124,6 +124,8 @@ SYMBOLS = [
     ("rocblas_amd_strsm_vbatched", _i32, _TRSM_VBATCHED),
     ("rocblas_amd_dtrsm_vbatched", _i32, _TRSM_VBATCHED),
     ("rocblas_amd_vbatched_set_tuning", _i32, [_i32, _i32]),
+    ("rocblas_amd_dtrsm_mixed", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32,
+                                       _vp, _i32, _i32, _i32, _i32, ctypes.c_double, _vp]),
     ("rocblas_amd_dsynth_tri", _i32, [_vp, _i32, _i32, _i32, ctypes.c_uint64, ctypes.c_double,
                                       ctypes.c_double, _vp, _i32]),
     ("rocblas_amd_ssynth_tri", _i32, [_vp, _i32, _i32, _i32, ctypes.c_uint64, ctypes.c_double,
@@ -277,6 +279,30 @@ def trsm_vbatched(handle, side, uplo, trans, diag, m_arr, n_arr, alpha, A_array,
             lda_arr, B_array, ldb_arr, batch)
     del keep
     return rocblas_status(st)
+
+
+class RefineInfo(ctypes.Structure):
+    """rocblas_amd_refine_info"""
+    _fields_ = [("iterations", ctypes.c_int32), ("backward_error", ctypes.c_double),
+                ("converged", ctypes.c_int32)]
+
+
+def trsm_mixed(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, block=0,
+               max_refine=10, tol=1e-13, alpha_ptr=None):
+    """rocblas_amd_dtrsm_mixed (FP32 trailing updates + FP64 refinement).
+    Returns (status, RefineInfo)."""
+    info = RefineInfo()
+    keep = None
+    a = alpha_ptr
+    if a is None and alpha is not None:
+        keep = ctypes.c_double(alpha)
+        a = ctypes.addressof(keep)
+    st = lib().rocblas_amd_dtrsm_mixed(_h(handle), int(SIDE.get(side, side)),
+                                       int(FILL.get(uplo, uplo)), int(OP.get(trans, trans)),
+                                       int(DIAG.get(diag, diag)), m, n, a, A, lda, B, ldb, 0,
+                                       block, max_refine, tol, ctypes.byref(info))
+    del keep
+    return rocblas_status(st), info
 
 
 def synth_tri(handle, uplo, dist, k, seed, A, lda, scale=1.0, poison=0.0, precision="d"):

@@ -134,24 +134,44 @@ rocblas_status launch_small_kmax(rocblas_handle h, const SmallArgs<T>& a, dim3 g
 // ---------------------------------------------------------------------------
 // GEMM (trailing updates)
 // ---------------------------------------------------------------------------
-template <typename T>
-struct GemmArgs
+// Triangular operand masks of the GEMM (residual B - op(A) X): op(A) / op(B)
+// restricted to its lower or upper triangle in global coordinates, with an
+// implicit unit diagonal when GT_UNIT is set.  Tiles wholly outside the
+// triangle are skipped.
+enum GemmTri : int
 {
-    int               M, N, K;
-    int               opA, opB; // 0 = none, 1 = transpose
-    BatchPtr<const T> A;
-    int               lda;
-    BatchPtr<const T> B;
-    int               ldb;
-    BatchPtr<T>       C;
-    int               ldc;
-    T                 alpha, beta;
-    const T*          skip_if_zero; // device scalar; == 0 -> no work (alpha == 0 in trsm)
-    int               tiles_m, tiles_n;
-    int               batch;
+    GT_A_LOWER = 1,
+    GT_A_UPPER = 2,
+    GT_B_LOWER = 4,
+    GT_B_UPPER = 8,
+    GT_UNIT    = 16
+};
+
+// Operand storage types may differ from the accumulator's: the mixed-precision
+// solve multiplies an FP32 copy of A with FP64 right-hand sides on FP32 MFMA
+// and accumulates into FP64 C.
+template <typename TA, typename TB = TA, typename TC = TA>
+struct GemmArgsT
+{
+    int                M, N, K;
+    int                opA, opB; // 0 = none, 1 = transpose
+    BatchPtr<const TA> A;
+    int                lda;
+    BatchPtr<const TB> B;
+    int                ldb;
+    BatchPtr<TC>       C;
+    int                ldc;
+    TC                 alpha, beta;
+    const TC*          skip_if_zero; // device scalar; == 0 -> no work (alpha == 0 in trsm)
+    int                tri;          // GemmTri bits, 0 = dense operands
+    int                tiles_m, tiles_n;
+    int                batch;
 };
 
 template <typename T>
-rocblas_status launch_gemm(rocblas_handle h, const GemmArgs<T>& a);
+using GemmArgs = GemmArgsT<T, T, T>;
+
+template <typename TA, typename TB, typename TC>
+rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a);
 
 } // namespace rbamd

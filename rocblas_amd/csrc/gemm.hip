@@ -2,11 +2,15 @@
 //
 //   C := alpha * op(A) * op(B) + beta * C   (column-major, batched)
 //
-// FP64 uses v_mfma_f64_16x16x4_f64, FP32 v_mfma_f32_16x16x4_f32.  Tiles are
+// FP64 uses v_mfma_f64_16x16x4_f64, FP32 v_mfma_f32_16x16x4_f32; operands
+// stored in a wider type than the MFMA's are rounded on their way into LDS
+// (the mixed-precision solve's FP32 updates of FP64 data).  Tiles are
 // staged through LDS (double buffered), with the next K step's global loads in
 // flight during the current step's MFMAs.  Workgroup ids are remapped so that
 // each XCD's L2 sees contiguous runs of tiles, grouped by row panel.
 #include "device_common.hpp"
+
+#include <type_traits>
 
 namespace rbamd
 {
@@ -18,8 +22,10 @@ namespace rbamd
 //   MFMAs of the current one; LDS is double buffered (one barrier per step).
 // ---------------------------------------------------------------------------
 
-template <typename T, int BM, int BN, int BK, int WGM, int WGN>
-__global__ void __launch_bounds__(WAVE * WGM * WGN) gemm_kernel(const GemmArgs<T> a)
+template <typename TA, typename TB, typename TC, typename T, bool TRI, int BM, int BN, int BK,
+          int WGM, int WGN>
+__global__ void __launch_bounds__(WAVE * WGM * WGN)
+    gemm_kernel(const GemmArgsT<TA, TB, TC> a)
 {
     using acc_t              = typename Mfma<T>::acc_t;
     constexpr int NT         = WAVE * WGM * WGN;
@@ -37,7 +43,7 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN) gemm_kernel(const GemmArgs<T
     __shared__ T As[2][BK * LDA_S];
     __shared__ T Bs[2][BK * LDB_S];
 
-    if(a.skip_if_zero && *a.skip_if_zero == T(0))
+    if(a.skip_if_zero && *a.skip_if_zero == TC(0))
         return;
 
     const int tid  = threadIdx.x;
@@ -64,11 +70,47 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN) gemm_kernel(const GemmArgs<T
     const int     m0    = tm * BM;
     const int     n0    = tn * BN;
 
+    // K range holding non-zero products (triangular operands)
+    int kbeg = 0, kend = a.K;
+    if constexpr(TRI)
+    {
+        if(a.tri & GT_A_LOWER)
+            kend = min(kend, m0 + BM);
+        if(a.tri & GT_A_UPPER)
+            kbeg = max(kbeg, m0);
+        if(a.tri & GT_B_LOWER)
+            kbeg = max(kbeg, n0);
+        if(a.tri & GT_B_UPPER)
+            kend = min(kend, n0 + BN);
+        kbeg = kbeg / BK * BK;
+    }
+    // op(A)(i, k) / op(B)(k, j) after the triangular mask
+    auto tri_a = [&](int i, int k, TA v) -> TA {
+        if constexpr(TRI)
+        {
+            if(((a.tri & GT_A_LOWER) && k > i) || ((a.tri & GT_A_UPPER) && k < i))
+                return TA(0);
+            if((a.tri & (GT_A_LOWER | GT_A_UPPER)) && (a.tri & GT_UNIT) && k == i)
+                return TA(1);
+        }
+        return v;
+    };
+    auto tri_b = [&](int k, int j, TB v) -> TB {
+        if constexpr(TRI)
+        {
+            if(((a.tri & GT_B_LOWER) && k < j) || ((a.tri & GT_B_UPPER) && k > j))
+                return TB(0);
+            if((a.tri & (GT_B_LOWER | GT_B_UPPER)) && (a.tri & GT_UNIT) && k == j)
+                return TB(1);
+        }
+        return v;
+    };
+
     for(int b = blockIdx.y; b < a.batch; b += gridDim.y)
     {
-        const T* Ap = a.A.get(b);
-        const T* Bp = a.B.get(b);
-        T*       Cp = a.C.get(b);
+        const TA* Ap = a.A.get(b);
+        const TB* Bp = a.B.get(b);
+        TC*       Cp = a.C.get(b);
 
         acc_t acc[TM][TN];
 #pragma unroll
@@ -96,10 +138,18 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN) gemm_kernel(const GemmArgs<T
                     i  = e / BK;
                 }
                 const int gi = m0 + i, gk = k0 + kk;
-                T         v  = T(0);
-                if(gi < a.M && gk < a.K)
-                    v = a.opA == 0 ? Ap[gi + int64_t(gk) * a.lda] : Ap[gk + int64_t(gi) * a.lda];
-                ra[s] = v;
+                TA        v  = TA(0);
+                if(gi < a.M && gk < kend)
+                {
+                    bool skip = false;
+                    if constexpr(TRI)
+                        skip = (a.tri & GT_UNIT) && (a.tri & (GT_A_LOWER | GT_A_UPPER)) && gi == gk;
+                    if(!skip)
+                        v = a.opA == 0 ? Ap[gi + int64_t(gk) * a.lda]
+                                       : Ap[gk + int64_t(gi) * a.lda];
+                    v = tri_a(gi, gk, v);
+                }
+                ra[s] = T(v);
             }
 #pragma unroll
             for(int s = 0; s < EB; ++s)
@@ -117,10 +167,18 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN) gemm_kernel(const GemmArgs<T
                     kk = e / BN;
                 }
                 const int gj = n0 + j, gk = k0 + kk;
-                T         v  = T(0);
-                if(gj < a.N && gk < a.K)
-                    v = a.opB == 0 ? Bp[gk + int64_t(gj) * a.ldb] : Bp[gj + int64_t(gk) * a.ldb];
-                rb[s] = v;
+                TB        v  = TB(0);
+                if(gj < a.N && gk < kend)
+                {
+                    bool skip = false;
+                    if constexpr(TRI)
+                        skip = (a.tri & GT_UNIT) && (a.tri & (GT_B_LOWER | GT_B_UPPER)) && gj == gk;
+                    if(!skip)
+                        v = a.opB == 0 ? Bp[gk + int64_t(gj) * a.ldb]
+                                       : Bp[gj + int64_t(gk) * a.ldb];
+                    v = tri_b(gk, gj, v);
+                }
+                rb[s] = T(v);
             }
         };
         auto store_tiles = [&](int buf) {
@@ -160,15 +218,16 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN) gemm_kernel(const GemmArgs<T
             }
         };
 
-        const int ktiles = (a.K + BK - 1) / BK;
-        load_tiles(0);
+        const int ktiles = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+        if(ktiles > 0)
+            load_tiles(kbeg);
         for(int kt = 0; kt < ktiles; ++kt)
         {
             const int buf = kt & 1;
             store_tiles(buf);
             __syncthreads();
             if(kt + 1 < ktiles)
-                load_tiles((kt + 1) * BK);
+                load_tiles(kbeg + (kt + 1) * BK);
             const T* as = As[buf];
             const T* bs = Bs[buf];
 #pragma unroll
@@ -205,9 +264,9 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN) gemm_kernel(const GemmArgs<T
                     const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lane, r);
                     if(row < a.M)
                     {
-                        T& cref = Cp[row + int64_t(col) * a.ldc];
-                        T  v    = a.alpha * acc[i][j][r];
-                        if(a.beta != T(0))
+                        TC& cref = Cp[row + int64_t(col) * a.ldc];
+                        TC  v    = a.alpha * TC(acc[i][j][r]);
+                        if(a.beta != TC(0))
                             v = fma(a.beta, cref, v);
                         cref = v;
                     }
@@ -217,22 +276,38 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN) gemm_kernel(const GemmArgs<T
     }
 }
 
-template <typename T, int BM, int BN, int BK, int WGM, int WGN>
-static rocblas_status launch_gemm_cfg(rocblas_handle h, GemmArgs<T> a)
+// MFMA type of an operand pair: FP64 only when both operands are FP64.
+template <typename TA, typename TB>
+using mfma_t = std::conditional_t<std::is_same_v<TA, double> && std::is_same_v<TB, double>,
+                                  double, float>;
+
+template <typename TA, typename TB, typename TC, int BM, int BN, int BK, int WGM, int WGN>
+static rocblas_status launch_gemm_cfg(rocblas_handle h, GemmArgsT<TA, TB, TC> a)
 {
+    using T           = mfma_t<TA, TB>;
     a.tiles_m         = (a.M + BM - 1) / BM;
     a.tiles_n         = (a.N + BN - 1) / BN;
     const int64_t nt  = int64_t(a.tiles_m) * a.tiles_n;
     if(nt > int64_t(0x7fffffff))
         return rocblas_status_invalid_size;
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WGM, WGN>), dim3(unsigned(nt), grid_batch(a.batch)),
-                       dim3(WAVE * WGM * WGN), 0, h->stream, a);
+    const dim3 grid(unsigned(nt), grid_batch(a.batch));
+    if(a.tri)
+    {
+        if constexpr(std::is_same_v<TA, TB> && std::is_same_v<TA, TC>)
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, T, true, BM, BN, BK, WGM, WGN>), grid,
+                               dim3(WAVE * WGM * WGN), 0, h->stream, a);
+        else
+            return rocblas_status_not_implemented;
+    }
+    else
+        hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, T, false, BM, BN, BK, WGM, WGN>), grid,
+                           dim3(WAVE * WGM * WGN), 0, h->stream, a);
     return launch_status();
 }
 
 // Pick the largest tile that still gives every CU work.
-template <typename T>
-rocblas_status launch_gemm(rocblas_handle h, const GemmArgs<T>& a)
+template <typename TA, typename TB, typename TC>
+rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
 {
     if(a.M <= 0 || a.N <= 0)
         return rocblas_status_success;
@@ -241,20 +316,21 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgs<T>& a)
         return int64_t((a.M + bm - 1) / bm) * ((a.N + bn - 1) / bn) * a.batch;
     };
     if(a.N <= 32)
-        return tiles(128, 32) >= cus ? launch_gemm_cfg<T, 128, 32, 16, 4, 1>(h, a)
-                                     : launch_gemm_cfg<T, 32, 32, 16, 2, 2>(h, a);
+        return tiles(128, 32) >= cus ? launch_gemm_cfg<TA, TB, TC, 128, 32, 16, 4, 1>(h, a)
+                                     : launch_gemm_cfg<TA, TB, TC, 32, 32, 16, 2, 2>(h, a);
     if(a.M <= 32)
-        return tiles(32, 128) >= cus ? launch_gemm_cfg<T, 32, 128, 16, 1, 4>(h, a)
-                                     : launch_gemm_cfg<T, 32, 32, 16, 2, 2>(h, a);
+        return tiles(32, 128) >= cus ? launch_gemm_cfg<TA, TB, TC, 32, 128, 16, 1, 4>(h, a)
+                                     : launch_gemm_cfg<TA, TB, TC, 32, 32, 16, 2, 2>(h, a);
     if(tiles(128, 128) >= cus)
-        return launch_gemm_cfg<T, 128, 128, 16, 2, 2>(h, a);
+        return launch_gemm_cfg<TA, TB, TC, 128, 128, 16, 2, 2>(h, a);
     if(tiles(64, 64) >= cus)
-        return launch_gemm_cfg<T, 64, 64, 16, 2, 2>(h, a);
-    return launch_gemm_cfg<T, 32, 32, 16, 2, 2>(h, a);
+        return launch_gemm_cfg<TA, TB, TC, 64, 64, 16, 2, 2>(h, a);
+    return launch_gemm_cfg<TA, TB, TC, 32, 32, 16, 2, 2>(h, a);
 }
 
-
-template rocblas_status launch_gemm<float>(rocblas_handle, const GemmArgs<float>&);
-template rocblas_status launch_gemm<double>(rocblas_handle, const GemmArgs<double>&);
+template rocblas_status launch_gemm(rocblas_handle, const GemmArgsT<float, float, float>&);
+template rocblas_status launch_gemm(rocblas_handle, const GemmArgsT<double, double, double>&);
+template rocblas_status launch_gemm(rocblas_handle, const GemmArgsT<float, double, double>&);
+template rocblas_status launch_gemm(rocblas_handle, const GemmArgsT<double, float, double>&);
 
 } // namespace rbamd

@@ -41,8 +41,9 @@ inline uint64_t stream_key(uint64_t seed, uint64_t tag)
     return splitmix_mix(seed + tag * kTagMult);
 }
 
-// 0: diagonal U[1,2], off-diagonal U[-1,1]/k; 1: off-diagonal N(0,1)/sqrt(k)
-// from Box-Muller on counters k*k + 2(i + j*k) and +1.
+// 0: diagonal U[1,2], off-diagonal U[-1,1]/k; 1: off-diagonal N(0,1)/sqrt(k),
+// normal number k*k + i + j*k of the stream, i.e. Box-Muller on uniform
+// counters 2(k*k + i + j*k) and that + 1.
 template <typename T>
 __global__ void __launch_bounds__(256) synth_tri_kernel(T* A, int k, int lda, bool upper,
                                                          int dist, uint64_t key, double scale,
@@ -65,7 +66,7 @@ __global__ void __launch_bounds__(256) synth_tri_kernel(T* A, int k, int lda, bo
                 v = ((2.0 * splitmix_u01(key, c) - 1.0) / double(k)) * scale;
             else
             {
-                const uint64_t base = uint64_t(k) * uint64_t(k) + 2 * c;
+                const uint64_t base = 2 * (uint64_t(k) * uint64_t(k) + c);
                 const double   u1   = splitmix_u01(key, base);
                 const double   u2   = splitmix_u01(key, base + 1);
                 v = sqrt(-2.0 * log1p(-u1)) * cos(2.0 * M_PI * u2) * inv_sqrt_k * scale;

@@ -307,7 +307,7 @@ struct Solver
             g.C   = p.B.shifted(int64_t(R2) * p.ldb);
             g.ldc = p.ldb;
         }
-        return launch_gemm<T>(h, g);
+        return launch_gemm(h, g);
     }
 
     rocblas_status solve(int a, int b)

@@ -1,0 +1,140 @@
+"""GPU tests of rocblas_amd_dtrsm_mixed (FP32 trailing updates, FP64 diagonal-
+block inverses, FP64 residual and iterative refinement) through the C ABI.
+
+The mixed form answers the same problem as rocblas_dtrsm
+(library/src/blas3/rocblas_trsm.cpp); results are checked against the oracle
+(the netlib loop nest of the clients' host cblas_dtrsm) and by the normwise
+backward error the workload demands (<= 1e-12); the call's own measurement
+of it must meet the tolerance it was given.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import rocblas_amd as rb
+from tests.gpu_util import GpuHandle, dev, host, say
+from tests.helpers import ALL_CASES, integer_kat, op_dense
+
+pytestmark = pytest.mark.gpu
+S = rb.rocblas_status
+
+
+@pytest.fixture(scope="module")
+def h():
+    assert torch.cuda.is_available()
+    oracle.build()
+    hd = GpuHandle()
+    yield hd
+    hd.close()
+
+
+def eta_np(side, uplo, trans, diag, alpha, A, B0, X):
+    k = A.shape[1]
+    op = op_dense(A, uplo, trans, diag, k)
+    R = alpha * B0 - (op @ X if side == "L" else X @ op)
+    n1 = lambda M: np.abs(M).sum(axis=0).max()
+    return n1(R) / (n1(op) * n1(X) + abs(alpha) * n1(B0))
+
+
+@pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_against_oracle_and_backward_error(h, side, uplo, trans, diag):
+    """Sizes with full and partial diagonal blocks (block 128): the refined result
+    meets eta <= 1e-12, agrees with the oracle, and the reported eta is the
+    measured one."""
+    rng = np.random.default_rng(200 + 2 * ALL_CASES.index((side, uplo, trans)) + (diag == "U"))
+    for m, n in [(300, 70), (256, 129), (90, 333)]:
+        say("mixed", side, uplo, trans, diag, m, n)
+        k = m if side == "L" else n
+        A = oracle.diag_dominant_triangular(k, uplo, rng, lda=k + 1)
+        if diag == "U":
+            oracle.make_unit_diagonal(A, uplo, k)
+        B = np.zeros((m + 2, n), order="F")
+        B[:m] = rng.uniform(-1, 1, (m, n))
+        alpha = -0.75
+        dA, dB = dev(A), dev(B)
+        st, info = rb.trsm_mixed(h, side, uplo, trans, diag, m, n, alpha, dA.data_ptr(), k + 1,
+                                 dB.data_ptr(), m + 2, block=128, max_refine=8, tol=1e-14)
+        assert st == S.success
+        torch.cuda.synchronize()
+        got = host(dB, B.shape)
+        assert np.all(got[m:] == 0.0)  # padding rows untouched
+        X = got[:m]
+        ref = B.copy(order="F")
+        oracle.trsm(side, uplo, trans, diag, m, n, alpha, A, ref)
+        e = eta_np(side, uplo, trans, diag, alpha, A, B[:m], X)
+        assert e <= 1e-12, e
+        # one leaf (k <= 128) is a pure FP64 product: no refinement needed
+        assert info.converged == 1 and info.iterations >= (1 if k > 128 else 0)
+        assert 0.0 <= info.backward_error <= 1e-14
+        assert np.abs(X - ref[:m]).max() <= 1e-11 * np.abs(ref[:m]).max()
+
+
+def test_integer_problem(h):
+    """Integer solution of an integer system (NaN in the unreferenced triangle):
+    the refined X rounds to the exact X and is within 1e-12 of it."""
+    rng = np.random.default_rng(5)
+    m, n = 260, 33
+    A, B, X = integer_kat("L", "U", "N", "N", m, n, rng)
+    dA, dB = dev(A), dev(B)
+    st, info = rb.trsm_mixed(h, "L", "U", "N", "N", m, n, 1.0, dA.data_ptr(), m, dB.data_ptr(),
+                             m, block=64)
+    assert st == S.success and info.converged == 1
+    torch.cuda.synchronize()
+    got = host(dB, B.shape)
+    assert np.array_equal(np.rint(got), X)
+    assert np.abs(got - X).max() <= 1e-12 * np.abs(X).max()
+
+
+def test_fixed_steps_and_block_sizes(h):
+    """tol <= 0 runs exactly max_refine steps (no measurement); the answer does
+    not depend on the block size beyond the refinement's accuracy."""
+    rng = np.random.default_rng(9)
+    m, n = 700, 40
+    A = oracle.diag_dominant_triangular(m, "L", rng, poison=0.0)
+    B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    outs = []
+    for nb in (64, 128, 256, 700, 0):
+        say("mixed block", nb)
+        dA, dB = dev(A), dev(B)
+        st, info = rb.trsm_mixed(h, "L", "L", "T", "N", m, n, 2.0, dA.data_ptr(), m,
+                                 dB.data_ptr(), m, block=nb, max_refine=2, tol=0.0)
+        assert st == S.success
+        assert info.iterations == 2 and info.backward_error == -1.0
+        torch.cuda.synchronize()
+        X = host(dB, B.shape)
+        assert eta_np("L", "L", "T", "N", 2.0, A, B, X) <= 1e-12
+        outs.append(X)
+    for X in outs[1:]:
+        assert np.abs(X - outs[0]).max() <= 1e-12 * np.abs(outs[0]).max()
+
+
+def test_argument_checks_and_alpha_zero(h):
+    m = n = 8
+    dA = torch.zeros(m * m, dtype=torch.float64, device="cuda")
+    dB = torch.ones(m * n, dtype=torch.float64, device="cuda")
+    L = rb.lib()
+    one = ctypes.c_double(1.0)
+    ap = ctypes.addressof(one)
+    f = lambda *a: L.rocblas_amd_dtrsm_mixed(*a)
+    assert f(None, 141, 121, 111, 131, m, n, ap, dA.data_ptr(), m, dB.data_ptr(), m, 0, 0, 3,
+             1e-13, None) == S.invalid_handle
+    assert f(h.ptr, 140, 121, 111, 131, m, n, ap, dA.data_ptr(), m, dB.data_ptr(), m, 0, 0, 3,
+             1e-13, None) == S.invalid_value
+    assert f(h.ptr, 141, 121, 111, 131, m, n, ap, dA.data_ptr(), m, dB.data_ptr(), m, 7, 0, 3,
+             1e-13, None) == S.invalid_value
+    assert f(h.ptr, 141, 121, 111, 131, m, n, ap, dA.data_ptr(), m - 1, dB.data_ptr(), m, 0, 0,
+             3, 1e-13, None) == S.invalid_size
+    assert f(h.ptr, 141, 121, 111, 131, m, n, ap, dA.data_ptr(), m, dB.data_ptr(), m, 0, -1, 3,
+             1e-13, None) == S.invalid_size
+    assert f(h.ptr, 141, 121, 111, 131, m, n, ap, None, m, dB.data_ptr(), m, 0, 0, 3, 1e-13,
+             None) == S.invalid_pointer
+    assert f(h.ptr, 141, 121, 111, 131, 0, n, ap, None, m, None, m, 0, 0, 3, 1e-13,
+             None) == S.success
+    st, info = rb.trsm_mixed(h, "L", "U", "N", "N", m, n, 0.0, 0, m, dB.data_ptr(), m)
+    assert st == S.success
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(dB).item() == 0
