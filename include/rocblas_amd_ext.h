@@ -66,6 +66,40 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_dtrsm_vbatched(rocblas_handle     
                                                              const rocblas_int*  ldb,
                                                              rocblas_int         batch_count);
 
+/* C := alpha op(A) op(B) + beta C with the MFMA kernel that carries the
+ * trailing updates of the blocked solve.  Same signature and argument rules
+ * as rocblas_{s,d}gemm (reference rocblas-functions.h:20047-20060); beta == 0
+ * does not read C.  Prefixed because only the GEMM shapes the solve uses are
+ * tuned. */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_sgemm(rocblas_handle    handle,
+                                                    rocblas_operation transA,
+                                                    rocblas_operation transB,
+                                                    rocblas_int       m,
+                                                    rocblas_int       n,
+                                                    rocblas_int       k,
+                                                    const float*      alpha,
+                                                    const float*      A,
+                                                    rocblas_int       lda,
+                                                    const float*      B,
+                                                    rocblas_int       ldb,
+                                                    const float*      beta,
+                                                    float*            C,
+                                                    rocblas_int       ldc);
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_dgemm(rocblas_handle    handle,
+                                                    rocblas_operation transA,
+                                                    rocblas_operation transB,
+                                                    rocblas_int       m,
+                                                    rocblas_int       n,
+                                                    rocblas_int       k,
+                                                    const double*     alpha,
+                                                    const double*     A,
+                                                    rocblas_int       lda,
+                                                    const double*     B,
+                                                    rocblas_int       ldb,
+                                                    const double*     beta,
+                                                    double*           C,
+                                                    rocblas_int       ldc);
+
 /* Low-precision form of the trailing updates of rocblas_amd_dtrsm_mixed. */
 typedef enum rocblas_amd_precision_form_
 {

@@ -124,6 +124,10 @@ SYMBOLS = [
     ("rocblas_amd_strsm_vbatched", _i32, _TRSM_VBATCHED),
     ("rocblas_amd_dtrsm_vbatched", _i32, _TRSM_VBATCHED),
     ("rocblas_amd_vbatched_set_tuning", _i32, [_i32, _i32]),
+    ("rocblas_amd_sgemm", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp,
+                                 _vp, _i32]),
+    ("rocblas_amd_dgemm", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp,
+                                 _vp, _i32]),
     ("rocblas_amd_dtrsm_mixed", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32,
                                        _vp, _i32, _i32, _i32, _i32, ctypes.c_double, _vp]),
     ("rocblas_amd_dsynth_tri", _i32, [_vp, _i32, _i32, _i32, ctypes.c_uint64, ctypes.c_double,
@@ -279,6 +283,16 @@ def trsm_vbatched(handle, side, uplo, trans, diag, m_arr, n_arr, alpha, A_array,
             lda_arr, B_array, ldb_arr, batch)
     del keep
     return rocblas_status(st)
+
+
+def gemm(handle, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, precision="d"):
+    """rocblas_amd_{s,d}gemm with host scalars."""
+    ct = ctypes.c_double if precision == "d" else ctypes.c_float
+    a, b = ct(alpha), ct(beta)
+    fn = lib().rocblas_amd_dgemm if precision == "d" else lib().rocblas_amd_sgemm
+    return rocblas_status(fn(_h(handle), int(OP.get(transA, transA)), int(OP.get(transB, transB)),
+                             m, n, k, ctypes.addressof(a), A, lda, B, ldb, ctypes.addressof(b), C,
+                             ldc))
 
 
 class RefineInfo(ctypes.Structure):

@@ -22,23 +22,24 @@ namespace rbamd
 //   MFMAs of the current one; LDS is double buffered (one barrier per step).
 // ---------------------------------------------------------------------------
 
-template <typename TA, typename TB, typename TC, typename T, bool TRI, int BM, int BN, int BK,
-          int WGM, int WGN>
+template <typename TA, typename TB, typename TC, typename T, bool TRI, int OPA, int OPB, int BM,
+          int BN, int BK, int WGM, int WGN>
 __global__ void __launch_bounds__(WAVE * WGM * WGN)
     gemm_kernel(const GemmArgsT<TA, TB, TC> a)
 {
-    using acc_t              = typename Mfma<T>::acc_t;
-    constexpr int NT         = WAVE * WGM * WGN;
-    constexpr int WTM        = BM / WGM;
-    constexpr int WTN        = BN / WGN;
-    constexpr int TM         = WTM / 16;
-    constexpr int TN         = WTN / 16;
-    constexpr int LDA_S      = BM + 2;
-    constexpr int LDB_S      = BN + 2;
-    constexpr int EA         = BM * BK / NT;
-    constexpr int EB         = BK * BN / NT;
+    using acc_t         = typename Mfma<T>::acc_t;
+    constexpr int NT    = WAVE * WGM * WGN;
+    constexpr int WTM   = BM / WGM;
+    constexpr int WTN   = BN / WGN;
+    constexpr int TM    = WTM / 16;
+    constexpr int TN    = WTN / 16;
+    constexpr int LDA_S = BM + 2;
+    constexpr int LDB_S = BN + 2;
+    constexpr int EA    = BM * BK / NT;
+    constexpr int EB    = BK * BN / NT;
     static_assert(EA * NT == BM * BK && EB * NT == BK * BN, "tile/threads mismatch");
     static_assert(TM >= 1 && TN >= 1 && BK % 4 == 0, "bad tile");
+    static_assert(NT % BM == 0 && NT % BN == 0 && NT % BK == 0, "load geometry");
 
     __shared__ T As[2][BK * LDA_S];
     __shared__ T Bs[2][BK * LDB_S];
@@ -84,27 +85,17 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
             kend = min(kend, n0 + BN);
         kbeg = kbeg / BK * BK;
     }
-    // op(A)(i, k) / op(B)(k, j) after the triangular mask
-    auto tri_a = [&](int i, int k, TA v) -> TA {
-        if constexpr(TRI)
-        {
-            if(((a.tri & GT_A_LOWER) && k > i) || ((a.tri & GT_A_UPPER) && k < i))
-                return TA(0);
-            if((a.tri & (GT_A_LOWER | GT_A_UPPER)) && (a.tri & GT_UNIT) && k == i)
-                return TA(1);
-        }
-        return v;
-    };
-    auto tri_b = [&](int k, int j, TB v) -> TB {
-        if constexpr(TRI)
-        {
-            if(((a.tri & GT_B_LOWER) && k < j) || ((a.tri & GT_B_UPPER) && k > j))
-                return TB(0);
-            if((a.tri & (GT_B_LOWER | GT_B_UPPER)) && (a.tri & GT_UNIT) && k == j)
-                return TB(1);
-        }
-        return v;
-    };
+    const int ktiles = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+
+    // Per-thread load geometry, fixed over K: element s of the A tile is
+    // (row ia(s), k offset ka(s)), of the B tile (k offset kb(s), column jb(s)),
+    // chosen so that consecutive threads read consecutive addresses.
+    auto ia = [&](int s) { return OPA == 0 ? tid % BM : tid / BK + s * (NT / BK); };
+    auto ka = [&](int s) { return OPA == 0 ? tid / BM + s * (NT / BM) : tid % BK; };
+    auto kb = [&](int s) { return OPB == 0 ? tid % BK : tid / BN + s * (NT / BN); };
+    auto jb = [&](int s) { return OPB == 0 ? tid / BK + s * (NT / BK) : tid % BN; };
+    const bool unitA = TRI && (a.tri & GT_UNIT) && (a.tri & (GT_A_LOWER | GT_A_UPPER));
+    const bool unitB = TRI && (a.tri & GT_UNIT) && (a.tri & (GT_B_LOWER | GT_B_UPPER));
 
     for(int b = blockIdx.y; b < a.batch; b += gridDim.y)
     {
@@ -121,104 +112,55 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
 
         T ra[EA], rb[EB];
 
+        // Branch-free tile loads: out-of-range indices are clamped to a valid
+        // element and the value replaced by zero (or by the triangular mask).
         auto load_tiles = [&](int k0) {
 #pragma unroll
             for(int s = 0; s < EA; ++s)
             {
-                const int e = tid + NT * s;
-                int       i, kk;
-                if(a.opA == 0)
+                const int  gi  = m0 + ia(s);
+                const int  gk  = k0 + ka(s);
+                const bool ok  = gi < a.M && gk < kend;
+                const int  gic = min(gi, a.M - 1);
+                const int  gkc = min(gk, kend - 1);
+                TA v = gload(Ap + (OPA == 0 ? gic + int64_t(gkc) * a.lda
+                                            : gkc + int64_t(gic) * a.lda));
+                if constexpr(TRI)
                 {
-                    i  = e % BM;
-                    kk = e / BM;
+                    const bool zero = ((a.tri & GT_A_LOWER) && gk > gi)
+                                      || ((a.tri & GT_A_UPPER) && gk < gi);
+                    v = zero ? TA(0) : (unitA && gk == gi ? TA(1) : v);
                 }
-                else
-                {
-                    kk = e % BK;
-                    i  = e / BK;
-                }
-                const int gi = m0 + i, gk = k0 + kk;
-                TA        v  = TA(0);
-                if(gi < a.M && gk < kend)
-                {
-                    bool skip = false;
-                    if constexpr(TRI)
-                        skip = (a.tri & GT_UNIT) && (a.tri & (GT_A_LOWER | GT_A_UPPER)) && gi == gk;
-                    if(!skip)
-                        v = a.opA == 0 ? Ap[gi + int64_t(gk) * a.lda]
-                                       : Ap[gk + int64_t(gi) * a.lda];
-                    v = tri_a(gi, gk, v);
-                }
-                ra[s] = T(v);
+                ra[s] = ok ? T(v) : T(0);
             }
 #pragma unroll
             for(int s = 0; s < EB; ++s)
             {
-                const int e = tid + NT * s;
-                int       j, kk;
-                if(a.opB == 0)
+                const int  gj  = n0 + jb(s);
+                const int  gk  = k0 + kb(s);
+                const bool ok  = gj < a.N && gk < kend;
+                const int  gjc = min(gj, a.N - 1);
+                const int  gkc = min(gk, kend - 1);
+                TB v = gload(Bp + (OPB == 0 ? gkc + int64_t(gjc) * a.ldb
+                                            : gjc + int64_t(gkc) * a.ldb));
+                if constexpr(TRI)
                 {
-                    kk = e % BK;
-                    j  = e / BK;
+                    const bool zero = ((a.tri & GT_B_LOWER) && gk < gj)
+                                      || ((a.tri & GT_B_UPPER) && gk > gj);
+                    v = zero ? TB(0) : (unitB && gk == gj ? TB(1) : v);
                 }
-                else
-                {
-                    j  = e % BN;
-                    kk = e / BN;
-                }
-                const int gj = n0 + j, gk = k0 + kk;
-                TB        v  = TB(0);
-                if(gj < a.N && gk < kend)
-                {
-                    bool skip = false;
-                    if constexpr(TRI)
-                        skip = (a.tri & GT_UNIT) && (a.tri & (GT_B_LOWER | GT_B_UPPER)) && gj == gk;
-                    if(!skip)
-                        v = a.opB == 0 ? Bp[gk + int64_t(gj) * a.ldb]
-                                       : Bp[gj + int64_t(gk) * a.ldb];
-                    v = tri_b(gk, gj, v);
-                }
-                rb[s] = T(v);
+                rb[s] = ok ? T(v) : T(0);
             }
         };
         auto store_tiles = [&](int buf) {
 #pragma unroll
             for(int s = 0; s < EA; ++s)
-            {
-                const int e = tid + NT * s;
-                int       i, kk;
-                if(a.opA == 0)
-                {
-                    i  = e % BM;
-                    kk = e / BM;
-                }
-                else
-                {
-                    kk = e % BK;
-                    i  = e / BK;
-                }
-                As[buf][kk * LDA_S + i] = ra[s];
-            }
+                As[buf][ka(s) * LDA_S + ia(s)] = ra[s];
 #pragma unroll
             for(int s = 0; s < EB; ++s)
-            {
-                const int e = tid + NT * s;
-                int       j, kk;
-                if(a.opB == 0)
-                {
-                    kk = e % BK;
-                    j  = e / BK;
-                }
-                else
-                {
-                    j  = e % BN;
-                    kk = e / BN;
-                }
-                Bs[buf][kk * LDB_S + j] = rb[s];
-            }
+                Bs[buf][kb(s) * LDB_S + jb(s)] = rb[s];
         };
 
-        const int ktiles = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
         if(ktiles > 0)
             load_tiles(kbeg);
         for(int kt = 0; kt < ktiles; ++kt)
@@ -226,8 +168,8 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
             const int buf = kt & 1;
             store_tiles(buf);
             __syncthreads();
-            if(kt + 1 < ktiles)
-                load_tiles(kbeg + (kt + 1) * BK);
+            // the load past the last tile reads clamped addresses and is discarded
+            load_tiles(kbeg + (kt + 1) * BK);
             const T* as = As[buf];
             const T* bs = Bs[buf];
 #pragma unroll
@@ -241,28 +183,28 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
 #pragma unroll
                 for(int j = 0; j < TN; ++j)
                     bf[j] = bs[kr * LDB_S + wn * WTN + j * 16 + (lane & 15)];
+                // operands swapped: the MFMA computes the transposed tile, so a
+                // lane's results run down a column of C (coalesced stores)
 #pragma unroll
                 for(int i = 0; i < TM; ++i)
 #pragma unroll
                     for(int j = 0; j < TN; ++j)
-                        acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
+                        acc[i][j] = Mfma<T>::run(bf[j], af[i], acc[i][j]);
             }
         }
 
-        // epilogue: lane holds column lane%16 and rows Mfma<T>::row(lane, r) of each tile
+        // epilogue: lane holds row lane%16 and columns Mfma<T>::row(lane, r) of each tile
 #pragma unroll
         for(int i = 0; i < TM; ++i)
+        {
+            const int row = m0 + wm * WTM + i * 16 + (lane & 15);
 #pragma unroll
             for(int j = 0; j < TN; ++j)
-            {
-                const int col = n0 + wn * WTN + j * 16 + (lane & 15);
-                if(col >= a.N)
-                    continue;
 #pragma unroll
                 for(int r = 0; r < 4; ++r)
                 {
-                    const int row = m0 + wm * WTM + i * 16 + Mfma<T>::row(lane, r);
-                    if(row < a.M)
+                    const int col = n0 + wn * WTN + j * 16 + Mfma<T>::row(lane, r);
+                    if(row < a.M && col < a.N)
                     {
                         TC& cref = Cp[row + int64_t(col) * a.ldc];
                         TC  v    = a.alpha * TC(acc[i][j][r]);
@@ -271,7 +213,7 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
                         cref = v;
                     }
                 }
-            }
+        }
         __syncthreads(); // LDS reuse across batch iterations
     }
 }
@@ -281,27 +223,69 @@ template <typename TA, typename TB>
 using mfma_t = std::conditional_t<std::is_same_v<TA, double> && std::is_same_v<TB, double>,
                                   double, float>;
 
+template <typename TA, typename TB, typename TC, bool TRI, int BM, int BN, int BK, int WGM,
+          int WGN>
+static void launch_gemm_ops(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a, dim3 grid)
+{
+    using T = mfma_t<TA, TB>;
+    // operand layouts the callers use: FP32-copy updates are op(A) x B
+    // (side left) or X x op(A) (side right)
+    constexpr bool any_ops = std::is_same_v<TA, TB>;
+    constexpr bool ta_ok   = any_ops || std::is_same_v<TA, float>;
+    constexpr bool tb_ok   = any_ops || std::is_same_v<TB, float>;
+    const dim3     block(WAVE * WGM * WGN);
+    if(a.opA == 0 && a.opB == 0)
+        hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, T, TRI, 0, 0, BM, BN, BK, WGM, WGN>), grid,
+                           block, 0, h->stream, a);
+    else if(a.opA == 1 && a.opB == 0)
+    {
+        if constexpr(ta_ok)
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, T, TRI, 1, 0, BM, BN, BK, WGM, WGN>), grid,
+                               block, 0, h->stream, a);
+    }
+    else if(a.opA == 0 && a.opB == 1)
+    {
+        if constexpr(tb_ok)
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, T, TRI, 0, 1, BM, BN, BK, WGM, WGN>), grid,
+                               block, 0, h->stream, a);
+    }
+    else
+    {
+        if constexpr(any_ops && !TRI)
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, T, TRI, 1, 1, BM, BN, BK, WGM, WGN>), grid,
+                               block, 0, h->stream, a);
+    }
+}
+
+template <typename TA, typename TB, typename TC>
+static bool gemm_ops_supported(const GemmArgsT<TA, TB, TC>& a)
+{
+    if constexpr(std::is_same_v<TA, TB>)
+        return !(a.tri && a.opA == 1 && a.opB == 1);
+    else if constexpr(std::is_same_v<TA, float>)
+        return a.opB == 0 && !a.tri;
+    else
+        return a.opA == 0 && !a.tri;
+}
+
 template <typename TA, typename TB, typename TC, int BM, int BN, int BK, int WGM, int WGN>
 static rocblas_status launch_gemm_cfg(rocblas_handle h, GemmArgsT<TA, TB, TC> a)
 {
-    using T           = mfma_t<TA, TB>;
-    a.tiles_m         = (a.M + BM - 1) / BM;
-    a.tiles_n         = (a.N + BN - 1) / BN;
-    const int64_t nt  = int64_t(a.tiles_m) * a.tiles_n;
+    a.tiles_m        = (a.M + BM - 1) / BM;
+    a.tiles_n        = (a.N + BN - 1) / BN;
+    const int64_t nt = int64_t(a.tiles_m) * a.tiles_n;
     if(nt > int64_t(0x7fffffff))
         return rocblas_status_invalid_size;
+    if(!gemm_ops_supported(a))
+        return rocblas_status_not_implemented;
     const dim3 grid(unsigned(nt), grid_batch(a.batch));
     if(a.tri)
     {
         if constexpr(std::is_same_v<TA, TB> && std::is_same_v<TA, TC>)
-            hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, T, true, BM, BN, BK, WGM, WGN>), grid,
-                               dim3(WAVE * WGM * WGN), 0, h->stream, a);
-        else
-            return rocblas_status_not_implemented;
+            launch_gemm_ops<TA, TB, TC, true, BM, BN, BK, WGM, WGN>(h, a, grid);
     }
     else
-        hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, T, false, BM, BN, BK, WGM, WGN>), grid,
-                           dim3(WAVE * WGM * WGN), 0, h->stream, a);
+        launch_gemm_ops<TA, TB, TC, false, BM, BN, BK, WGM, WGN>(h, a, grid);
     return launch_status();
 }
 
