@@ -184,6 +184,68 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_ssynth_rhs(rocblas_handle handle,
                                                          float*         B,
                                                          rocblas_int    ldb);
 
+/* ---- serving loop ---------------------------------------------------------
+ * A server owns one device arena of hbm_budget_bytes (its admission budget)
+ * and `streams` streams with a handle each.  rocblas_amd_server_run replays a
+ * trace of requests, each the solve L X = B (side left, lower, no transpose,
+ * non-unit, alpha 1) with a synthetic factor L (uniform distribution of
+ * rocblas_amd_{s,d}synth_tri, kept resident per factor_id and evicted least
+ * recently used when the arena is full) and a synthetic B: requests are
+ * admitted FIFO once they have arrived and their memory fits, up to
+ * max_inflight at once; those with m <= small_k are solved together in one
+ * _vbatched launch per precision, larger ones by rocblas_{s,d}trsm, over the
+ * streams in turn, so new work overlaps work in flight.  latency[i] is the
+ * host time from arrival to observed completion (-1 if rejected).  No
+ * counterpart in the reference. */
+typedef struct rocblas_amd_server_* rocblas_amd_server;
+
+typedef struct rocblas_amd_request_
+{
+    int64_t  factor_id;   /* requests with one id share one resident factor */
+    uint64_t factor_seed; /* seed of the factor when it is generated */
+    uint64_t rhs_seed;    /* seed of B */
+    double   arrival;     /* seconds after the start of the run; non-decreasing */
+    int32_t  m;           /* triangle size */
+    int32_t  n;           /* right-hand sides */
+    int32_t  precision;   /* 0: FP64, 1: FP32 */
+    int32_t  keep;        /* keep X on the device for rocblas_amd_server_result */
+} rocblas_amd_request;
+
+typedef struct rocblas_amd_serve_stats_
+{
+    int64_t completed;
+    int64_t rejected; /* larger than the whole arena */
+    int64_t launches;
+    int64_t batched_launches;
+    int64_t factor_hits;
+    int64_t factor_misses;
+    int64_t evictions;
+    double  wall_seconds;
+    double  p50_latency; /* seconds */
+    double  p99_latency;
+    double  max_latency;
+    double  mean_latency;
+    double  peak_arena_bytes;
+} rocblas_amd_serve_stats;
+
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_create(rocblas_amd_server* server,
+                                                            size_t      hbm_budget_bytes,
+                                                            rocblas_int streams,
+                                                            rocblas_int max_inflight,
+                                                            rocblas_int small_k);
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_destroy(rocblas_amd_server server);
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_run(rocblas_amd_server         server,
+                                                         const rocblas_amd_request* requests,
+                                                         int64_t                    count,
+                                                         double*                    latency,
+                                                         rocblas_amd_serve_stats*   stats);
+/* Copies the kept X of request `index` of the last run to host memory X
+ * (m x n, leading dimension ldx, in the request's precision). */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_result(rocblas_amd_server server,
+                                                            int64_t            index,
+                                                            void*              X,
+                                                            rocblas_int        ldx);
+
 /* Overrides the size-bucket tuning of the _vbatched solves for this process:
  * instances with k <= nb16_max_k use 16-row blocks (larger ones 32-row
  * blocks), and the persistent launch has waves_per_cu wavefronts per compute
