@@ -12,7 +12,12 @@ instance) over its average launch duration measured with HIP events on the
 launch stream; cpu_baseline = the oracle's netlib loop nest, OpenMP over
 instances, on this host's cores (a bounded sample of the same workload).
 
+Further workloads (single 512x256 solve, 16384^2 plain and mixed-precision
+solves, a 4096-instance variable-size batch, the Poisson serving loop) are
+measured after the main line by bench_legs.py and reported under "workloads".
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--legs all|none|single,large,vbatch,serve] [--serve-seconds S]
 """
 import argparse
 import json
@@ -109,6 +114,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="all",
+                    help="further workloads: all, none, or a comma list of "
+                         "single,large,vbatch,serve")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain device copy of 248,320,000 bytes (the side-L "
                          "launch's algorithmic bytes), so a rocprofv3 --pmc run of this "
@@ -210,11 +219,35 @@ def main():
                      "algorithmic_bytes_per_launch": BATCH * algo_bytes("L"),
                      "avg_launch_us": float(left_us.mean())},
     }
+    import oracle
+    oracle.build()
     if not args.no_cpu:
-        import oracle
-        oracle.build()
         out["cpu_baseline"] = cpu_baseline(oracle, data, Bsrc)
     h.close()
+    legs = {"single", "large", "vbatch", "serve"} if args.legs == "all" else \
+        set() if args.legs == "none" else set(args.legs.split(","))
+    if legs:
+        import bench_legs as bl
+        del data
+        torch.cuda.empty_cache()
+        vendor = bl.vendor_lib(torch)
+        work = {}
+        runs = [("single_512x256", "single", lambda: bl.leg_single(torch, rb, synth, oracle, vendor)),
+                ("large_16384", "large", lambda: bl.leg_large(torch, rb, oracle, vendor)),
+                ("variable_batch_4096", "vbatch", lambda: bl.leg_variable_batch(torch, rb, vendor)),
+                ("serving_poisson_2000rps", "serve",
+                 lambda: bl.leg_serving(torch, rb, duration=args.serve_seconds))]
+        for name, key, fn in runs:
+            if key not in legs:
+                continue
+            print(f"[bench] {name} ...", file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            try:
+                work[name] = fn()
+            except Exception as e:  # report, keep the main line
+                work[name] = {"error": f"{type(e).__name__}: {e}"}
+            work[name]["leg_seconds"] = time.perf_counter() - t0
+        out["workloads"] = work
     if eta > 1e-12:
         print(f"backward error {eta:.3e} above 1e-12", file=sys.stderr)
     print(json.dumps(out))

@@ -1,0 +1,309 @@
+"""Additional measured workloads of bench.py (one JSON object each).
+
+* single_512x256   -- rocblas_dtrsm L/L/N/N m=512 n=256 alpha=1, A lower with
+                      diagonal U[1,2] and off-diagonal U[-1,1]/m, B U[-1,1],
+                      seed 0: per-call latency p50/p99, TFLOP/s, backward error.
+* large_16384      -- m=n=16384 L/U/N/N, off-diagonal N(0,1)/sqrt(m): the plain
+                      FP64 blocked solve and the mixed form (FP32 MFMA trailing
+                      updates, FP64 diagonal-block inverses, FP64 residual and
+                      refinement to <= 1e-12), both credited with m*n*k flops.
+* variable_batch   -- 4096 instances, side L lower, m log-uniform in [8, 512],
+                      n uniform in {1..64}: one graph-captured vbatched launch vs
+                      a per-instance loop of rocblas_dtrsm.
+* serving          -- Poisson arrivals at 2000 req/s, m log-uniform in
+                      [16, 4096], n in {1..128}, FP64/FP32, 10% factor reuse,
+                      200 GB admission budget: p50/p99 latency and solves/s.
+
+Where the vendor librocblas that ships with torch is present, its
+rocblas_dtrsm is timed beside ours through its C API (a comparator only).
+"""
+import ctypes
+import os
+import time
+
+import numpy as np
+
+FP64_PEAK_TF = 78.6
+
+
+def _events(torch, n):
+    return [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            for _ in range(n)]
+
+
+def vendor_lib(torch):
+    path = os.path.join(os.path.dirname(torch.__file__), "lib", "librocblas.so")
+    if not os.path.exists(path):
+        return None
+    V = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    V.rocblas_dtrsm.argtypes = [vp] + [ctypes.c_int] * 6 + [vp, vp, ctypes.c_int, vp,
+                                                             ctypes.c_int]
+    h = vp()
+    if V.rocblas_create_handle(ctypes.byref(h)) != 0:
+        return None
+    V.rocblas_set_stream(h, vp(torch.cuda.current_stream().cuda_stream))
+    return V, h
+
+
+def eta_device(torch, side, uplo, A, B0, X, alpha=1.0):
+    """Normwise backward error with a plain PyTorch float64 residual on the device."""
+    T = torch.triu(A) if uplo == "U" else torch.tril(A)
+    R = alpha * B0 - (T @ X if side == "L" else X @ T)
+    n1 = lambda Z: Z.abs().sum(0).max()
+    return float(n1(R) / (n1(T) * n1(X) + abs(alpha) * n1(B0)))
+
+
+def leg_single(torch, rb, synth, oracle, vendor):
+    m, n = 512, 256
+    A = synth.tri_uniform(m, "L", 0)
+    B = synth.rhs_uniform(m, n, 0)
+    dA = torch.from_numpy(A.ravel(order="F").copy()).cuda()
+    dB0 = torch.from_numpy(B.ravel(order="F").copy()).cuda()
+    h = rb.Handle()
+    rb.check(h.set_stream(torch.cuda.current_stream().cuda_stream))
+    bufs = [dB0.clone() for _ in range(8)]
+    out = {}
+    for lib in ("ours", "vendor"):
+        if lib == "vendor" and vendor is None:
+            continue
+        ts = []
+        for r in range(105):
+            dB = bufs[r % 8]
+            dB.copy_(dB0)
+            ev = _events(torch, 1)[0]
+            ev[0].record()
+            if lib == "ours":
+                st = rb.trsm(h, "L", "L", "N", "N", m, n, 1.0, dA.data_ptr(), m, dB.data_ptr(), m)
+            else:
+                one = ctypes.c_double(1.0)
+                V, vh = vendor
+                st = V.rocblas_dtrsm(vh, 141, 122, 111, 131, m, n, ctypes.addressof(one),
+                                     ctypes.c_void_p(dA.data_ptr()), m,
+                                     ctypes.c_void_p(dB.data_ptr()), m)
+            ev[1].record()
+            torch.cuda.synchronize()
+            assert st == 0, (lib, st)
+            if r >= 5:
+                ts.append(ev[0].elapsed_time(ev[1]) * 1e3)
+        ts = np.array(ts)
+        X = bufs[104 % 8].view(n, m).T
+        e = eta_device(torch, "L", "L", dA.view(m, m).T, dB0.view(n, m).T, X)
+        out[lib] = {"p50_us": float(np.percentile(ts, 50)), "p99_us": float(np.percentile(ts, 99)),
+                    "tflops": m * n * m / (np.percentile(ts, 50) * 1e-6) / 1e12,
+                    "backward_error": e}
+    h.close()
+    # oracle on one host core
+    Bw = B.copy(order="F")
+    t0 = time.perf_counter()
+    oracle.trsm("L", "L", "N", "N", m, n, 1.0, A, Bw)
+    t = time.perf_counter() - t0
+    return {"workload": "dtrsm L/L/N/N m=512 n=256 alpha=1 (diag U[1,2], off-diag U[-1,1]/m)",
+            "ours": out["ours"], "vendor_rocblas": out.get("vendor"),
+            "cpu_oracle_us": t * 1e6, "cpu_cores": 1}
+
+
+def leg_large(torch, rb, oracle, vendor, m=16384, reps=3):
+    n = m
+    h = rb.Handle()
+    rb.check(h.set_stream(torch.cuda.current_stream().cuda_stream))
+    dA = torch.empty(m * m, dtype=torch.float64, device="cuda")
+    dB0 = torch.empty(m * n, dtype=torch.float64, device="cuda")
+    rb.check(rb.synth_tri(h, "U", "normal", m, 0, dA.data_ptr(), m))
+    rb.check(rb.synth_rhs(h, m, n, 0, dB0.data_ptr(), m))
+    dB = torch.empty_like(dB0)
+    flops = float(m) * n * m
+    res = {}
+
+    def timed(fn):
+        ts = []
+        for r in range(reps + 1):
+            dB.copy_(dB0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            if r:
+                ts.append(time.perf_counter() - t0)
+        return min(ts), out
+
+    t, _ = timed(lambda: rb.check(rb.trsm(h, "L", "U", "N", "N", m, n, 1.0, dA.data_ptr(), m,
+                                          dB.data_ptr(), m)))
+    Am, B0m = dA.view(m, m).T, dB0.view(n, m).T
+    e = eta_device(torch, "L", "U", Am, B0m, dB.view(n, m).T)
+    res["plain_fp64"] = {"ms": t * 1e3, "tflops": flops / t / 1e12,
+                         "frac_fp64_peak": flops / t / 1e12 / FP64_PEAK_TF, "backward_error": e}
+    info_box = {}
+
+    def mixed():
+        st, info = rb.trsm_mixed(h, "L", "U", "N", "N", m, n, 1.0, dA.data_ptr(), m,
+                                 dB.data_ptr(), m, max_refine=10, tol=5e-13)
+        rb.check(st)
+        info_box["info"] = info
+
+    t, _ = timed(mixed)
+    e = eta_device(torch, "L", "U", Am, B0m, dB.view(n, m).T)
+    info = info_box["info"]
+    res["mixed_fp32_refined"] = {"ms": t * 1e3, "tflops": flops / t / 1e12,
+                                 "speedup_vs_plain": res["plain_fp64"]["ms"] / (t * 1e3),
+                                 "refinement_steps": info.iterations,
+                                 "reported_backward_error": info.backward_error,
+                                 "backward_error": e}
+    if vendor is not None:
+        V, vh = vendor
+        one = ctypes.c_double(1.0)
+        t, _ = timed(lambda: V.rocblas_dtrsm(vh, 141, 121, 111, 131, m, n, ctypes.addressof(one),
+                                             ctypes.c_void_p(dA.data_ptr()), m,
+                                             ctypes.c_void_p(dB.data_ptr()), m))
+        res["vendor_rocblas_fp64"] = {"ms": t * 1e3, "tflops": flops / t / 1e12}
+    # CPU oracle on a bounded column sample, scaled to the full width
+    A_host = dA.view(m, m).T.cpu().numpy()
+    A_host = np.asfortranarray(A_host)
+    cols = 8
+    Bs = np.asfortranarray(dB0.view(n, m).T[:, :cols].cpu().numpy())
+    t0 = time.perf_counter()
+    oracle.trsm_omp("L", "U", "N", "N", m, cols, 1.0, A_host, Bs, oracle.num_threads())
+    tc = time.perf_counter() - t0
+    res["cpu_oracle"] = {"seconds_full_estimate": tc * n / cols, "cores": oracle.num_threads(),
+                         "sample": f"{cols} of {n} right-hand sides, {tc:.1f} s, scaled"}
+    h.close()
+    del dA, dB0, dB, A_host
+    torch.cuda.empty_cache()
+    return dict({"workload": "dtrsm L/U/N/N m=n=16384, off-diagonal N(0,1)/sqrt(m)"}, **res)
+
+
+def leg_variable_batch(torch, rb, vendor, batch=4096, seed=0):
+    from rocblas_amd import synth
+    key = synth.stream_key(seed, synth.TAG_SIZES)
+    u = synth.uniform(key, 0, 2 * batch)
+    ms = np.clip(np.exp(np.log(8) + u[:batch] * (np.log(513) - np.log(8))).astype(np.int64), 8,
+                 512)
+    ns = np.clip((1 + u[batch:] * 64).astype(np.int64), 1, 64)
+    h = rb.Handle()
+    s = torch.cuda.current_stream()
+    rb.check(h.set_stream(s.cuda_stream))
+    offA = np.concatenate([[0], np.cumsum(ms * ms)])
+    offB = np.concatenate([[0], np.cumsum(ms * ns)])
+    dA = torch.empty(int(offA[-1]), dtype=torch.float64, device="cuda")
+    dB0 = torch.empty(int(offB[-1]), dtype=torch.float64, device="cuda")
+    for i in range(batch):
+        m, n = int(ms[i]), int(ns[i])
+        rb.check(rb.synth_tri(h, "L", "uniform", m, seed + i, dA.data_ptr() + 8 * int(offA[i]), m))
+        rb.check(rb.synth_rhs(h, m, n, seed + i, dB0.data_ptr() + 8 * int(offB[i]), m))
+    dB = dB0.clone()
+    i32 = lambda v: torch.from_numpy(np.asarray(v, dtype=np.int32)).cuda()
+    dm, dn = i32(ms), i32(ns)
+    pa = torch.tensor([dA.data_ptr() + 8 * int(o) for o in offA[:-1]], dtype=torch.int64).cuda()
+    pb = torch.tensor([dB.data_ptr() + 8 * int(o) for o in offB[:-1]], dtype=torch.int64).cuda()
+    flops = float(np.sum(ms.astype(float) ** 2 * ns))
+
+    def vb():
+        return rb.trsm_vbatched(h, "L", "L", "N", "N", dm.data_ptr(), dn.data_ptr(), 1.0,
+                                pa.data_ptr(), dm.data_ptr(), pb.data_ptr(), dm.data_ptr(), batch)
+
+    # graph-captured launch (plan + persistent solve), replayed
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    gs = torch.cuda.Stream()
+    hg = rb.Handle()
+    rb.check(hg.set_stream(gs.cuda_stream))
+    with torch.cuda.graph(g, stream=gs):
+        rb.check(rb.trsm_vbatched(hg, "L", "L", "N", "N", dm.data_ptr(), dn.data_ptr(), 1.0,
+                                  pa.data_ptr(), dm.data_ptr(), pb.data_ptr(), dm.data_ptr(),
+                                  batch))
+    ts = []
+    for r in range(11):
+        dB.copy_(dB0)
+        torch.cuda.synchronize()
+        ev = _events(torch, 1)[0]
+        ev[0].record()
+        g.replay()
+        ev[1].record()
+        torch.cuda.synchronize()
+        if r:
+            ts.append(ev[0].elapsed_time(ev[1]) * 1e-3)
+    t_graph = float(np.median(ts))
+    # correctness: per-instance backward error on a sample
+    worst = 0.0
+    for i in range(0, batch, 97):
+        m, n = int(ms[i]), int(ns[i])
+        A = dA[int(offA[i]):int(offA[i + 1])].view(m, m).T
+        B0 = dB0[int(offB[i]):int(offB[i + 1])].view(n, m).T
+        X = dB[int(offB[i]):int(offB[i + 1])].view(n, m).T
+        worst = max(worst, eta_device(torch, "L", "L", A, B0, X))
+    # per-instance loop of the plain entry point
+    def loop(lib):
+        dB.copy_(dB0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        one = ctypes.c_double(1.0)
+        for i in range(batch):
+            m, n = int(ms[i]), int(ns[i])
+            a = dA.data_ptr() + 8 * int(offA[i])
+            b = dB.data_ptr() + 8 * int(offB[i])
+            if lib == "ours":
+                st = rb.trsm(h, "L", "L", "N", "N", m, n, 1.0, a, m, b, m)
+            else:
+                V, vh = vendor
+                st = V.rocblas_dtrsm(vh, 141, 122, 111, 131, m, n, ctypes.addressof(one),
+                                     ctypes.c_void_p(a), m, ctypes.c_void_p(b), m)
+            assert st == 0
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    loop("ours")
+    t_loop = min(loop("ours") for _ in range(2))
+    res = {"workload": "4096 instances, side L lower, m log-uniform [8,512], n uniform {1..64}",
+           "instances": batch, "gflop": flops / 1e9,
+           "vbatched_graph": {"ms": t_graph * 1e3, "solves_per_s": batch / t_graph,
+                              "tflops": flops / t_graph / 1e12, "backward_error_sampled": worst},
+           "loop_of_dtrsm_ours": {"ms": t_loop * 1e3, "solves_per_s": batch / t_loop},
+           "speedup_vs_loop": t_loop / t_graph}
+    if vendor is not None:
+        loop("vendor")
+        t_v = min(loop("vendor") for _ in range(2))
+        res["loop_of_dtrsm_vendor"] = {"ms": t_v * 1e3, "solves_per_s": batch / t_v}
+        res["speedup_vs_vendor_loop"] = t_v / t_graph
+    hg.close()
+    h.close()
+    del dA, dB0, dB
+    torch.cuda.empty_cache()
+    return res
+
+
+def leg_serving(torch, rb, duration=60.0, rate=2000.0, budget=200 << 30, check=24):
+    from rocblas_amd import serve
+    reqs = serve.poisson_trace(rate, duration, seed=0, keep_every=max(1, int(rate * duration)
+                                                                      // check))
+    srv = serve.Server(budget=budget, streams=4, max_inflight=512)
+    try:
+        lat, st = srv.run(reqs)
+        worst = {0: 0.0, 1: 0.0}
+        checked = 0
+        for i in range(len(reqs)):
+            r = reqs[i]
+            if not r.keep or r.m > 2048:
+                continue
+            X = srv.result(i).astype(np.float64)
+            A, B = serve.host_problem(r)
+            A, B = A.astype(np.float64), B.astype(np.float64)
+            L = np.tril(A)
+            R = B - L @ X
+            n1 = lambda Z: np.abs(Z).sum(axis=0).max()
+            e = n1(R) / (n1(L) * n1(X) + n1(B))
+            worst[r.precision] = max(worst[r.precision], e)
+            checked += 1
+    finally:
+        srv.close()
+    return {"workload": f"Poisson {rate:.0f} req/s x {duration:.0f} s, m log-uniform [16,4096], "
+                        f"n uniform {{1..128}}, FP64/FP32, 10% factor reuse, "
+                        f"{budget / 2**30:.0f} GiB budget",
+            "requests": len(reqs), "completed": st.completed, "rejected": st.rejected,
+            "solves_per_s": st.completed / st.wall_seconds, "wall_s": st.wall_seconds,
+            "p50_ms": st.p50_latency * 1e3, "p99_ms": st.p99_latency * 1e3,
+            "mean_ms": st.mean_latency * 1e3, "max_ms": st.max_latency * 1e3,
+            "launches": st.launches, "batched_launches": st.batched_launches,
+            "factor_hits": st.factor_hits, "factor_misses": st.factor_misses,
+            "evictions": st.evictions, "peak_arena_gib": st.peak_arena_bytes / 2**30,
+            "checked_results": checked, "max_backward_error_fp64": worst[0],
+            "max_backward_error_fp32": worst[1]}

@@ -127,6 +127,17 @@ constexpr int slot_elems(int kmax)
     return (kmax * (kmax + 1) / 2) | 1;
 }
 
+// Single-launch panel solve for SMALL_K < k <= kPanelMaxK (panel_trsm.hip):
+// one workgroup per 16 vectors, same SmallArgs (K, nvec, lda, ldb, flags,
+// alpha); `units` is ignored (the batch count is passed separately).
+constexpr int kPanelMaxK = 512;
+
+template <typename T>
+bool panel_fits(rocblas_handle h, int K);
+
+template <typename T>
+rocblas_status launch_panel(rocblas_handle h, const SmallArgs<T>& a, int batch);
+
 // Launch one small-solve grid; defined per KMAX in small_k*.hip.
 template <typename T, int KMAX>
 rocblas_status launch_small_kmax(rocblas_handle h, const SmallArgs<T>& a, dim3 grid, size_t lds);

@@ -310,11 +310,32 @@ struct Solver
         return launch_gemm(h, g);
     }
 
+    // SMALL_K < k <= kPanelMaxK: one panel launch solves the whole range
+    rocblas_status panel_leaf(int a, int b)
+    {
+        const int    K  = b - a;
+        const int    r0 = real_lo(a, b);
+        SmallArgs<T> s{};
+        s.A          = p.A.shifted(int64_t(r0) + int64_t(r0) * p.lda);
+        s.B          = p.B.shifted(sideL ? int64_t(r0) : int64_t(r0) * p.ldb);
+        s.K          = K;
+        s.nvec       = sideL ? p.n : p.m;
+        s.lda        = p.lda;
+        s.ldb        = p.ldb;
+        s.flags      = (tA ? SF_TA : 0) | (upperA ? SF_UPPER : 0) | (rev ? SF_REV : 0)
+                  | (p.diag == rocblas_diagonal_unit ? SF_UNIT : 0) | (sideL ? SF_COLS : 0);
+        s.alpha_host = T(1);
+        s.alpha_dev  = p.alpha_dev; // only the alpha == 0 guard
+        return launch_panel<T>(h, s, p.batch);
+    }
+
     rocblas_status solve(int a, int b)
     {
         const int K = b - a;
         if(K <= SMALL_K)
             return leaf(a, b);
+        if(panel_fits<T>(h, K))
+            return panel_leaf(a, b);
         const int nb = (K + SMALL_K - 1) / SMALL_K;
         const int K1 = ((nb + 1) / 2) * SMALL_K;
         rocblas_status st = solve(a, a + K1);
@@ -359,6 +380,24 @@ rocblas_status trsm_launch(rocblas_handle h, const TrsmProblem<T>& p)
         s.alpha_host = p.alpha_host;
         s.alpha_dev  = p.alpha_dev;
         return launch_small<T>(h, s, p.batch);
+    }
+
+    if(panel_fits<T>(h, K))
+    {
+        // one launch: the panel kernel scales by alpha itself
+        SmallArgs<T> s{};
+        s.A          = p.A;
+        s.B          = p.B;
+        s.K          = K;
+        s.nvec       = sideL ? p.n : p.m;
+        s.lda        = p.lda;
+        s.ldb        = p.ldb;
+        s.flags      = (tA ? SF_TA : 0) | (upperA ? SF_UPPER : 0) | (lowerM ? 0 : SF_REV)
+                  | (p.diag == rocblas_diagonal_unit ? SF_UNIT : 0) | (sideL ? SF_COLS : 0)
+                  | SF_SCALE;
+        s.alpha_host = p.alpha_host;
+        s.alpha_dev  = p.alpha_dev;
+        return launch_panel<T>(h, s, p.batch);
     }
 
     if(p.alpha_dev || p.alpha_host != T(1))

@@ -92,26 +92,25 @@ def test_mix_matches_alone_and_oracle():
 
 
 def test_small_budget_evicts_and_stays_correct():
-    """An arena of 24 MiB holds only a few factors: the loop evicts the least
-    recently used idle factors, regenerates them on later use, and results
-    are unchanged."""
+    """An arena of 8 MiB holds two or three factors (m in [300, 600]): the loop
+    evicts the least recently used idle factors, regenerates them on later use,
+    and results are unchanged."""
     rows = mixed_rows(2, 30, 0.0)
-    for r in rows:
-        r["m"] = min(r["m"], 600)
-    # one factor id keeps a single shape
     first = {}
-    for r in rows:
-        f = first.setdefault(r["factor_id"], (r["m"], r["precision"]))
-        r["m"], r["precision"] = f
+    for i, r in enumerate(rows):
+        f = first.setdefault(r["factor_id"], (300 + r["m"] % 300, r["precision"]))
+        r["m"], r["precision"] = f  # one factor id keeps a single shape
+        r["keep"] = 1 if i % 6 == 0 else 0
     reqs = serve.make_requests(rows)
-    srv = serve.Server(budget=24 << 20, streams=2, max_inflight=8)
+    budget = 8 << 20
+    srv = serve.Server(budget=budget, streams=2, max_inflight=8)
     try:
         say("serve small budget")
         lat, st = srv.run(reqs)
-        assert st.completed == len(rows)
+        assert st.completed == len(rows) and st.rejected == 0
         assert st.evictions >= 1
-        assert st.peak_arena_bytes <= 24 << 20
-        for i in range(len(rows)):
+        assert st.peak_arena_bytes <= budget
+        for i in range(0, len(rows), 6):
             check_result(reqs[i], srv.result(i))
     finally:
         srv.close()
