@@ -127,6 +127,9 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
                                             : gkc + int64_t(gic) * a.lda));
                 if constexpr(TRI)
                 {
+                    // keep the load unconditional: without the pin the compiler
+                    // sinks it under the mask as a divergent branch per element
+                    asm volatile("" : "+v"(v));
                     const bool zero = ((a.tri & GT_A_LOWER) && gk > gi)
                                       || ((a.tri & GT_A_UPPER) && gk < gi);
                     v = zero ? TA(0) : (unitA && gk == gi ? TA(1) : v);
@@ -145,6 +148,7 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
                                             : gjc + int64_t(gkc) * a.ldb));
                 if constexpr(TRI)
                 {
+                    asm volatile("" : "+v"(v));
                     const bool zero = ((a.tri & GT_B_LOWER) && gk < gj)
                                       || ((a.tri & GT_B_UPPER) && gk > gj);
                     v = zero ? TB(0) : (unitB && gk == gj ? TB(1) : v);
