@@ -173,33 +173,15 @@ def leg_large(torch, rb, oracle, vendor, m=16384, reps=3):
 
 
 def leg_variable_batch(torch, rb, vendor, batch=4096, seed=0):
-    from rocblas_amd import synth
-    key = synth.stream_key(seed, synth.TAG_SIZES)
-    u = synth.uniform(key, 0, 2 * batch)
-    ms = np.clip(np.exp(np.log(8) + u[:batch] * (np.log(513) - np.log(8))).astype(np.int64), 8,
-                 512)
-    ns = np.clip((1 + u[batch:] * 64).astype(np.int64), 1, 64)
+    from rocblas_amd import workloads
     h = rb.Handle()
     s = torch.cuda.current_stream()
     rb.check(h.set_stream(s.cuda_stream))
-    offA = np.concatenate([[0], np.cumsum(ms * ms)])
-    offB = np.concatenate([[0], np.cumsum(ms * ns)])
-    dA = torch.empty(int(offA[-1]), dtype=torch.float64, device="cuda")
-    dB0 = torch.empty(int(offB[-1]), dtype=torch.float64, device="cuda")
-    for i in range(batch):
-        m, n = int(ms[i]), int(ns[i])
-        rb.check(rb.synth_tri(h, "L", "uniform", m, seed + i, dA.data_ptr() + 8 * int(offA[i]), m))
-        rb.check(rb.synth_rhs(h, m, n, seed + i, dB0.data_ptr() + 8 * int(offB[i]), m))
-    dB = dB0.clone()
-    i32 = lambda v: torch.from_numpy(np.asarray(v, dtype=np.int32)).cuda()
-    dm, dn = i32(ms), i32(ns)
-    pa = torch.tensor([dA.data_ptr() + 8 * int(o) for o in offA[:-1]], dtype=torch.int64).cuda()
-    pb = torch.tensor([dB.data_ptr() + 8 * int(o) for o in offB[:-1]], dtype=torch.int64).cuda()
-    flops = float(np.sum(ms.astype(float) ** 2 * ns))
-
-    def vb():
-        return rb.trsm_vbatched(h, "L", "L", "N", "N", dm.data_ptr(), dn.data_ptr(), 1.0,
-                                pa.data_ptr(), dm.data_ptr(), pb.data_ptr(), dm.data_ptr(), batch)
+    w = workloads.variable_batch(h, batch, 8, 512, 1, 64, seed)
+    ms, ns, offA, offB = w["ms"], w["ns"], w["offA"], w["offB"]
+    dA, dB0, dB = w["dA"], w["dB0"], w["dB"]
+    dm, dn, pa, pb = w["m"], w["n"], w["pa"], w["pb"]
+    flops = w["flops"]
 
     # graph-captured launch (plan + persistent solve), replayed
     torch.cuda.synchronize()
@@ -266,7 +248,7 @@ def leg_variable_batch(torch, rb, vendor, batch=4096, seed=0):
         res["speedup_vs_vendor_loop"] = t_v / t_graph
     hg.close()
     h.close()
-    del dA, dB0, dB
+    del dA, dB0, dB, w
     torch.cuda.empty_cache()
     return res
 

@@ -30,9 +30,10 @@ typedef enum rocblas_amd_synth_dist_
  * op(A_i) X_i = alpha B_i (side left) or X_i op(A_i) = alpha B_i (side right)
  * with its own m[i], n[i], lda[i], ldb[i].  m, n, lda, ldb, A and B are device
  * arrays of batch_count entries; alpha is one scalar (host or device per the
- * pointer mode).  All instances are solved by one persistent launch (after a
- * one-workgroup planning launch that orders them by cost); the call allocates
- * nothing beyond the handle workspace (4 bytes per instance + 64) and can be
+ * pointer mode).  A one-workgroup planning launch orders the work by cost and
+ * two persistent launches solve it (k <= 512 with the MFMA panel solve, larger
+ * triangles one wavefront each); the call allocates nothing beyond the handle
+ * workspace (36 bytes per instance + 64; batch_count < 2^28) and can be
  * captured in a HIP graph.  No counterpart in the reference: rocBLAS callers
  * loop over rocblas_dtrsm.  The per-instance sizes are device data, so they
  * are checked on the device: an instance with m[i] <= 0, n[i] <= 0,
@@ -246,14 +247,17 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_result(rocblas_amd_server s
                                                             void*              X,
                                                             rocblas_int        ldx);
 
-/* Overrides the size-bucket tuning of the _vbatched solves for this process:
- * instances with k <= nb16_max_k use 16-row blocks (larger ones 32-row
- * blocks), and the persistent launch has waves_per_cu wavefronts per compute
- * unit (1..32).  0 restores the built-in value of either.  Used by the
- * autotuner (rocblas_amd/autotune_vbatched.py); not thread-safe against
- * concurrent _vbatched calls.  No counterpart in the reference. */
+/* Overrides the tuning of the _vbatched solves for this process: for k > 512
+ * instances (one wavefront each), k <= nb16_max_k uses 16-row blocks (else
+ * 32-row blocks) and the launch keeps waves_per_cu wavefronts per compute
+ * unit (1..32); instances with k <= 512 are split into up to panel_groups
+ * work items (1..8) of the panel solve.  0 restores the built-in value of
+ * any of them.  Used by the autotuner (rocblas_amd/autotune_vbatched.py); not
+ * thread-safe against concurrent _vbatched calls.  No counterpart in the
+ * reference. */
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb16_max_k,
-                                                                  rocblas_int waves_per_cu);
+                                                                  rocblas_int waves_per_cu,
+                                                                  rocblas_int panel_groups);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,232 @@
+// panel_trsm.hpp -- the panel solve shared by the single-launch k <= 512
+// path (panel_trsm.hip) and the persistent variable-size batch (vbatched.hip).
+//
+// panel_solve() solves 16 right-hand-side vectors [v0, v0 + 16) of one
+// instance with all 256 threads of the workgroup.  Canonical form as in
+// small_trsm.hpp: forward substitution L y = c on the canonical lower triangle
+// (side L: vectors are columns of B, M = op(A); side R: vectors are rows of B,
+// M = op(A)^T; SF_REV reverses the index).  The triangle is walked in 64-row
+// blocks I, left-looking:
+//
+//   c_I = scale * b_I - sum_{J<I} L_IJ y_J    (FP64/FP32 MFMA, y_J from LDS)
+//   y_I = L_II^{-1} c_I                       (substitution, DPP broadcasts)
+//
+// Four wavefronts: in the update each owns 16 rows of the block (one 16x16
+// MFMA tile over the panel's 16 vectors); in the substitution each owns 4
+// vectors, a 16-lane row per vector, lane g holding rows g, g+16, g+32, g+48,
+// and the solved value of row j is broadcast from lane j%16 with one
+// v_mov_dpp row_newbcast.  Tiles of L are staged through LDS with coalesced
+// loads (either storage order of A), the next tile's loads in flight during
+// the current tile's MFMAs.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace rbamd
+{
+namespace panel
+{
+constexpr int PNB   = 64;       // rows per block
+constexpr int PNC   = 16;       // vectors per panel
+constexpr int PLDA  = PNB + 1;  // LDS row stride of a staged tile
+constexpr int PLDY  = PNC + 1;  // LDS row stride of the solved panel
+constexpr int PNT   = 256;      // threads
+constexpr int PEPT  = PNB * PNB / PNT; // tile elements per thread
+
+template <int N, typename T>
+__device__ __forceinline__ T bcast(T x)
+{
+    return __builtin_amdgcn_mov_dpp(x, 0x150 + N, 0xf, 0xf, true);
+}
+
+// One step j of the substitution (compile-time j for the DPP lane select).
+template <typename T, int J>
+__device__ __forceinline__ void panel_step(T (&c)[4], const T* Ls, const T* rdiag, int g)
+{
+    constexpr int q  = J / 16;
+    constexpr int jl = J % 16;
+    // the owner lane (g == jl) computes x; every lane of the row receives it
+    const T xo = c[q] * rdiag[J];
+    const T x  = bcast<jl>(xo);
+    if(g == jl)
+        c[q] = x;
+#pragma unroll
+    for(int r = 0; r < 4; ++r)
+        c[r] = fma(-Ls[(g + 16 * r) * PLDA + J], x, c[r]);
+}
+
+template <typename T, int J>
+struct PanelSolve
+{
+    static __device__ __forceinline__ void run(T (&c)[4], const T* Ls, const T* rdiag, int g)
+    {
+        PanelSolve<T, J - 1>::run(c, Ls, rdiag, g);
+        panel_step<T, J - 1>(c, Ls, rdiag, g);
+    }
+};
+template <typename T>
+struct PanelSolve<T, 0>
+{
+    static __device__ __forceinline__ void run(T (&)[4], const T*, const T*, int) {}
+};
+
+// LDS bytes of one workgroup solving triangles of size up to K
+template <typename T>
+__host__ __device__ constexpr size_t lds_bytes(int K)
+{
+    return sizeof(T) * (size_t(PNB) * PLDA + size_t(PNB) * PNC + PNB + size_t(K) * PLDY);
+}
+
+// B := 0 for the panel's vectors (alpha == 0; A is not referenced)
+template <typename T>
+__device__ void panel_zero(T* Bp, int K, int nvec, int ldb, int flags, int v0)
+{
+    const bool rev  = flags & SF_REV;
+    const bool cols = flags & SF_COLS;
+    for(int e = threadIdx.x; e < K * PNC; e += PNT)
+    {
+        const int i = e % K, v = v0 + e / K;
+        const int r = rev ? K - 1 - i : i;
+        if(v < nvec)
+            gstore(Bp + (cols ? int64_t(r) + int64_t(v) * ldb : int64_t(v) + int64_t(r) * ldb),
+                   T(0));
+    }
+}
+
+template <typename T>
+__device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, int lda, int ldb,
+                            int flags, T scale, int v0, unsigned char* smem)
+{
+    T* Ls    = reinterpret_cast<T*>(smem); // PNB x PLDA staged tile
+    T* Cs    = Ls + PNB * PLDA;            // PNB x PNC accumulated update
+    T* rdiag = Cs + PNB * PNC;             // PNB reciprocal diagonal
+    T* Ys    = rdiag + PNB;                // K x PLDY solved panel
+
+    const int  tid  = threadIdx.x;
+    const int  lane = tid & (WAVE - 1);
+    const int  w    = tid / WAVE;
+    const bool tA   = flags & SF_TA;
+    const bool rev  = flags & SF_REV;
+    const bool unit = flags & SF_UNIT;
+    const bool cols = flags & SF_COLS;
+    const int  nblk = (K + PNB - 1) / PNB;
+
+    // substitution layout: vector vl = 4w + lane/16, rows g + 16r of a block
+    const int  g   = lane & 15;
+    const int  vl  = 4 * w + (lane >> 4);
+    const int  vec = v0 + vl;
+    const bool vok = vec < nvec;
+
+    auto boff = [&](int i, int v) -> int64_t {
+        const int r = rev ? K - 1 - i : i;
+        return cols ? int64_t(r) + int64_t(v) * ldb : int64_t(v) + int64_t(r) * ldb;
+    };
+
+    // Tile (I, J) of canonical L, element s of this thread: canonical
+    // (i, j) chosen so that consecutive threads read consecutive addresses.
+    auto tile_ij = [&](int s, int& i, int& j) {
+        const int e = tid + PNT * s;
+        if(!tA)
+        {
+            i = e % PNB; // contiguous real rows (reversed when rev)
+            j = e / PNB;
+        }
+        else
+        {
+            j = e % PNB; // A^T: contiguous along canonical columns
+            i = e / PNB;
+        }
+    };
+    T    reg[PEPT];
+    auto load_tile = [&](int I, int J) {
+#pragma unroll
+        for(int s = 0; s < PEPT; ++s)
+        {
+            int i, j;
+            tile_ij(s, i, j);
+            const int  gi = I * PNB + i, gj = J * PNB + j;
+            const bool in = gi < K && gj <= gi && !(unit && gi == gj);
+            const int  ci = in ? gi : 0, cj = in ? gj : 0;
+            const int  mi = rev ? K - 1 - ci : ci, mj = rev ? K - 1 - cj : cj;
+            const T    v  = gload(Ap + (tA ? int64_t(mj) + int64_t(mi) * lda
+                                          : int64_t(mi) + int64_t(mj) * lda));
+            reg[s] = in ? v : T(0);
+        }
+    };
+    auto store_tile = [&]() {
+#pragma unroll
+        for(int s = 0; s < PEPT; ++s)
+        {
+            int i, j;
+            tile_ij(s, i, j);
+            Ls[i * PLDA + j] = reg[s];
+        }
+    };
+
+    // tile sequence: (0,0), (1,0), (1,1), (2,0), ... ; (I, I) is the diagonal block
+    load_tile(0, 0);
+    for(int I = 0; I < nblk; ++I)
+    {
+        // ---- c_I accumulation: sum_{J<I} L_IJ y_J on MFMA ----
+        typename Mfma<T>::acc_t acc{0, 0, 0, 0};
+        for(int J = 0; J < I; ++J)
+        {
+            __syncthreads(); // previous readers of Ls are done
+            store_tile();
+            __syncthreads();
+            load_tile(I, J + 1); // next: (I, J+1), the diagonal block when J+1 == I
+#pragma unroll
+            for(int k4 = 0; k4 < PNB / 4; ++k4)
+            {
+                const int kk = 4 * k4 + (lane >> 4);
+                const T   av = Ls[(16 * w + (lane & 15)) * PLDA + kk];
+                const T   yv = Ys[(J * PNB + kk) * PLDY + (lane & 15)];
+                acc          = Mfma<T>::run(av, yv, acc);
+            }
+        }
+        __syncthreads();
+        // diagonal block (I, I): reciprocal diagonal out, zero diagonal in
+        store_tile();
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+            Cs[(16 * w + Mfma<T>::row(lane, r)) * PNC + (lane & 15)] = acc[r];
+        __syncthreads();
+        if(tid < PNB)
+        {
+            const int gi = I * PNB + tid;
+            const T   d  = Ls[tid * PLDA + tid];
+            rdiag[tid]   = (gi < K && !unit) ? T(1) / d : T(1);
+            Ls[tid * PLDA + tid] = T(0);
+        }
+        __syncthreads();
+        // prefetch the first tile of the next block during the substitution
+        if(I + 1 < nblk)
+            load_tile(I + 1, 0);
+
+        // ---- y_I = L_II^{-1} c_I ----
+        T c[4];
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+        {
+            const int  gi = I * PNB + g + 16 * r;
+            const bool ok = vok && gi < K;
+            const T    bv = gload(Bp + (ok ? boff(gi, vec) : 0));
+            c[r]          = ok ? scale * bv - Cs[(g + 16 * r) * PNC + vl] : T(0);
+        }
+        PanelSolve<T, PNB>::run(c, Ls, rdiag, g);
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+        {
+            const int gi = I * PNB + g + 16 * r;
+            if(gi < K)
+            {
+                Ys[gi * PLDY + vl] = c[r];
+                if(vok)
+                    gstore(Bp + boff(gi, vec), c[r]);
+            }
+        }
+    }
+    __syncthreads(); // LDS reuse by the caller's next panel
+}
+} // namespace panel
+} // namespace rbamd

@@ -1,5 +1,5 @@
-// vbatched.hip -- variable-size batched triangular solve in one persistent
-// launch (plus a one-workgroup planning launch), graph-capturable.
+// vbatched.hip -- variable-size batched triangular solve in one planning
+// launch plus two persistent launches, graph-capturable.
 //
 //   rocblas_amd_{s,d}trsm_vbatched(handle, side, uplo, transA, diag,
 //                                  m[], n[], alpha, A[], lda[], B[], ldb[], batch)
@@ -9,23 +9,27 @@
 // counterpart: rocBLAS has no variable-size batched trsm, and its callers loop
 // over rocblas_dtrsm (one to thirty launches per instance).
 //
-// Plan kernel (1 workgroup): counting sort of the instances by cost bucket
-// (floor(log2(k^2 * nvec))), largest first, into an order array in the handle
-// workspace, and reset of the work counter (a plain store: no memset node in a
-// graph).
+// Plan kernel (1 workgroup): every valid instance with k <= 512 becomes up to
+// kVbPanelGroups work items, each a strided share of the instance's 16-vector
+// panels; the items are counting-sorted by cost bucket
+// (floor(log2(k^2 * vectors))), largest first.  Instances with k > 512 go to a
+// second list.  Both work counters are reset with plain stores (no memset
+// node in a graph).
 //
-// Solve kernel (persistent, one wavefront per workgroup): each wavefront
-// takes the next instance from the counter and solves it on its own, 64
-// right-hand-side vectors at a time (lane = vector), with the canonical
-// forward-substitution form used by the small kernels (small_trsm.hpp):
-// blocked, left-looking over NB-row blocks of the canonical triangle,
+// Panel kernel (persistent, 256 threads, one workgroup per CU): each
+// workgroup takes the next item and solves its panels with the MFMA panel
+// solve of panel_trsm.hpp.
+//
+// Wave kernel (persistent, one wavefront per workgroup) for k > 512: each
+// wavefront takes the next instance and solves it on its own, 64
+// right-hand-side vectors at a time (lane = vector): blocked, left-looking
+// over NB-row blocks of the canonical triangle,
 //    c_I = alpha * b_I - sum_{J<I} L_IJ y_J ;  y_I = L_II^{-1} c_I
-// with each L block staged through LDS (coalesced along A's columns) and
-// read as LDS broadcasts; solved blocks y_J are re-read from B (L2-resident).
-// NB is 16 or 32 per instance, chosen by the size-bucket table in
-// vbatched_tuning.hpp (written by rocblas_amd/autotune_vbatched.py).
-// Results do not depend on the order in which wavefronts take instances.
-#include "device_common.hpp"
+// with each L block staged through LDS and read as LDS broadcasts.  NB is 16
+// or 32 by the size-bucket table in vbatched_tuning.hpp (written by
+// rocblas_amd/autotune_vbatched.py).
+// Results do not depend on the order in which workers take items.
+#include "panel_trsm.hpp"
 #include "vbatched_tuning.hpp"
 
 #include "../../include/rocblas_amd_ext.h"
@@ -48,17 +52,22 @@ struct VbArgs
     int                flags; // SF_TA / SF_UPPER / SF_REV / SF_UNIT / SF_COLS
     T                  alpha_host;
     const T*           alpha_dev;
-    int*               order;   // workspace: batch ints
-    int*               counter; // workspace: 1 int
-    int                nb16_max_k; // k <= this: 16-row blocks, else 32
+    int*               items;   // workspace: panel work items (inst * 8 + group, groups * batch)
+    int*               order;   // workspace: k > 512 instances (batch ints)
+    int*               counter; // workspace: [0] panel items taken, [1] large taken,
+                                //            [2] panel items, [3] large instances
+    int                nb16_max_k; // wave kernel: k <= this: 16-row blocks, else 32
+    int                groups;     // work items per instance of the panel kernel (1..8)
 };
 
 // Process-wide tuning values; start from the generated table and can be
 // overridden at run time (rocblas_amd_vbatched_set_tuning) by the autotuner.
 struct VbTuning
 {
-    int nb16_max_k   = kVbNb16MaxK;
-    int waves_per_cu = kVbWavesPerCu;
+    int nb16_max_k      = kVbNb16MaxK;
+    int waves_per_cu    = kVbWavesPerCu;
+    int groups          = kVbPanelGroups;
+    int panel_wgs_per_cu = 1;
 };
 static VbTuning g_vb_tuning;
 
@@ -71,39 +80,110 @@ __device__ __forceinline__ int vb_cost_bucket(long long k, long long nv)
     return VB_BUCKETS - 1 - min(lg, VB_BUCKETS - 1); // descending cost
 }
 
+// instance i: triangle size k, vectors nv, and whether it is solvable
+template <typename T>
+__device__ __forceinline__ bool vb_shape(const VbArgs<T>& a, int i, int& k, int& nv)
+{
+    const bool cols = a.flags & SF_COLS;
+    const int  m = a.m[i], n = a.n[i];
+    k  = cols ? m : n;
+    nv = cols ? n : m;
+    return m > 0 && n > 0 && a.lda[i] >= k && a.ldb[i] >= m;
+}
+
+template <typename T>
+__device__ __forceinline__ int vb_groups(const VbArgs<T>& a, int nv)
+{
+    const int panels = (nv + panel::PNC - 1) / panel::PNC;
+    return min(panels, a.groups);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(1024) vb_plan_kernel(const VbArgs<T> a)
 {
     __shared__ int hist[VB_BUCKETS];
     __shared__ int offs[VB_BUCKETS];
-    const bool     cols = a.flags & SF_COLS;
+    __shared__ int nlarge;
     for(int i = threadIdx.x; i < VB_BUCKETS; i += blockDim.x)
         hist[i] = 0;
+    if(threadIdx.x == 0)
+        nlarge = 0;
     __syncthreads();
     for(int i = threadIdx.x; i < a.batch; i += blockDim.x)
     {
-        const int k  = cols ? a.m[i] : a.n[i];
-        const int nv = cols ? a.n[i] : a.m[i];
-        atomicAdd(&hist[vb_cost_bucket(k, nv)], 1);
+        int k, nv;
+        if(!vb_shape(a, i, k, nv))
+            continue;
+        if(k <= kPanelMaxK)
+        {
+            const int G   = vb_groups(a, nv);
+            const int per = ((nv + panel::PNC - 1) / panel::PNC + G - 1) / G;
+            atomicAdd(&hist[vb_cost_bucket(k, min(nv, per * panel::PNC))], G);
+        }
+        else
+            a.order[atomicAdd(&nlarge, 1)] = i;
     }
     __syncthreads();
     if(threadIdx.x == 0)
     {
-        int s = 0;
+        int t = 0;
         for(int b = 0; b < VB_BUCKETS; ++b)
         {
-            offs[b] = s;
-            s += hist[b];
+            offs[b] = t;
+            t += hist[b];
         }
-        *a.counter = 0;
+        a.counter[0] = 0;
+        a.counter[1] = 0;
+        a.counter[2] = t;
+        a.counter[3] = nlarge;
     }
     __syncthreads();
     for(int i = threadIdx.x; i < a.batch; i += blockDim.x)
     {
-        const int k   = cols ? a.m[i] : a.n[i];
-        const int nv  = cols ? a.n[i] : a.m[i];
-        const int pos = atomicAdd(&offs[vb_cost_bucket(k, nv)], 1);
-        a.order[pos]  = i;
+        int k, nv;
+        if(!vb_shape(a, i, k, nv) || k > kPanelMaxK)
+            continue;
+        const int G   = vb_groups(a, nv);
+        const int per = ((nv + panel::PNC - 1) / panel::PNC + G - 1) / G;
+        const int pos = atomicAdd(&offs[vb_cost_bucket(k, min(nv, per * panel::PNC))], G);
+        for(int g = 0; g < G; ++g)
+            a.items[pos + g] = i * 8 + g;
+    }
+}
+
+// Panel kernel: persistent workgroups over the panel work items.
+template <typename T>
+__global__ void __launch_bounds__(panel::PNT) vb_panel_kernel(const VbArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char vb_smem[];
+    __shared__ int item_s;
+    const T        alpha = a.alpha_dev ? *a.alpha_dev : a.alpha_host;
+    const int      total = a.counter[2];
+    for(;;)
+    {
+        if(threadIdx.x == 0)
+            item_s = atomicAdd(&a.counter[0], 1);
+        __syncthreads();
+        const int item = item_s;
+        __syncthreads();
+        if(item >= total)
+            break;
+        const int inst = a.items[item] / 8;
+        const int g    = a.items[item] % 8;
+        int       k, nv;
+        vb_shape(a, inst, k, nv);
+        const int G      = vb_groups(a, nv);
+        const int panels = (nv + panel::PNC - 1) / panel::PNC;
+        const T*  Ap     = a.A[inst];
+        T*        Bp     = a.B[inst];
+        for(int p = g; p < panels; p += G)
+        {
+            if(alpha == T(0))
+                panel::panel_zero<T>(Bp, k, nv, a.ldb[inst], a.flags, p * panel::PNC);
+            else
+                panel::panel_solve<T>(Ap, Bp, k, nv, a.lda[inst], a.ldb[inst], a.flags, alpha,
+                                      p * panel::PNC, vb_smem);
+        }
     }
 }
 
@@ -230,14 +310,15 @@ __global__ void __launch_bounds__(WAVE) vb_solve_kernel(const VbArgs<T> a)
     const T    alpha = a.alpha_dev ? *a.alpha_dev : a.alpha_host;
     const bool cols  = a.flags & SF_COLS;
 
+    const int  total = a.counter[3];
     for(;;)
     {
         if(threadIdx.x == 0)
-            item_s = atomicAdd(a.counter, 1);
+            item_s = atomicAdd(&a.counter[1], 1);
         __syncthreads();
         const int item = item_s;
         __syncthreads();
-        if(item >= a.batch)
+        if(item >= total)
             break;
         const int inst = a.order[item];
         const int m    = a.m[inst];
@@ -280,6 +361,16 @@ rocblas_status vbatched_launch(rocblas_handle h, VbArgs<T> a)
     rocblas_status st = launch_status();
     if(st != rocblas_status_success)
         return st;
+    const size_t lds = panel::lds_bytes<T>(kPanelMaxK);
+    static const hipError_t opt = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&vb_panel_kernel<T>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    (void)opt;
+    (void)hipGetLastError();
+    const int wgs = std::min(a.batch * a.groups, h->num_cus * g_vb_tuning.panel_wgs_per_cu);
+    hipLaunchKernelGGL(vb_panel_kernel<T>, dim3(wgs), dim3(panel::PNT), lds, h->stream, a);
+    if((st = launch_status()) != rocblas_status_success)
+        return st;
     const int waves = std::min(a.batch, h->num_cus * g_vb_tuning.waves_per_cu);
     hipLaunchKernelGGL(vb_solve_kernel<T>, dim3(waves), dim3(WAVE), 0, h->stream, a);
     return launch_status();
@@ -313,9 +404,10 @@ rocblas_status vb_entry(rocblas_handle     handle,
         return rocblas_status_invalid_value;
     if(diag != rocblas_diagonal_non_unit && diag != rocblas_diagonal_unit)
         return rocblas_status_invalid_value;
-    if(batch_count < 0)
-        return rocblas_status_invalid_size;
-    const size_t ws = roundup_ws(size_t(batch_count) * sizeof(int)) + 64;
+    if(batch_count < 0 || batch_count >= (1 << 28))
+        return rocblas_status_invalid_size; // work items encode instance * 8 + group
+    const size_t ws = 64 + roundup_ws(size_t(batch_count) * sizeof(int))
+                      + roundup_ws(size_t(batch_count) * 8 * sizeof(int));
     if(handle->size_query)
     {
         if(!batch_count || roundup_ws(ws) <= handle->query_size)
@@ -355,8 +447,11 @@ rocblas_status vb_entry(rocblas_handle     handle,
     else
         a.alpha_dev = alpha;
     a.nb16_max_k = g_vb_tuning.nb16_max_k;
+    a.groups     = g_vb_tuning.groups;
     a.counter    = static_cast<int*>(lease.ptr);
-    a.order   = reinterpret_cast<int*>(static_cast<char*>(lease.ptr) + 64);
+    a.order      = reinterpret_cast<int*>(static_cast<char*>(lease.ptr) + 64);
+    a.items      = reinterpret_cast<int*>(static_cast<char*>(lease.ptr) + 64
+                                          + roundup_ws(size_t(batch_count) * sizeof(int)));
     return vbatched_launch<T>(handle, a);
 }
 } // namespace
@@ -396,13 +491,17 @@ extern "C" {
 RBAMD_VB(rocblas_amd_strsm_vbatched, float)
 RBAMD_VB(rocblas_amd_dtrsm_vbatched, double)
 
-rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb16_max_k, rocblas_int waves_per_cu)
+rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb16_max_k,
+                                               rocblas_int waves_per_cu,
+                                               rocblas_int panel_groups)
 {
-    if(nb16_max_k < 0 || waves_per_cu < 0 || waves_per_cu > 32)
+    if(nb16_max_k < 0 || waves_per_cu < 0 || waves_per_cu > 32 || panel_groups < 0
+       || panel_groups > 8)
         return rocblas_status_invalid_value;
     const VbTuning def;
     g_vb_tuning.nb16_max_k   = nb16_max_k ? nb16_max_k : def.nb16_max_k;
     g_vb_tuning.waves_per_cu = waves_per_cu ? waves_per_cu : def.waves_per_cu;
+    g_vb_tuning.groups       = panel_groups ? panel_groups : def.groups;
     return rocblas_status_success;
 }
 }
