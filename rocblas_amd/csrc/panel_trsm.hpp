@@ -137,20 +137,22 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
             i = e / PNB;
         }
     };
+    // raw values of the tile in flight; the triangle mask is applied when the
+    // tile is written to LDS, so no load is predicated on it
     T    reg[PEPT];
+    int  tI = 0, tJ = 0;
     auto load_tile = [&](int I, int J) {
+        tI = I;
+        tJ = J;
 #pragma unroll
         for(int s = 0; s < PEPT; ++s)
         {
             int i, j;
             tile_ij(s, i, j);
-            const int  gi = I * PNB + i, gj = J * PNB + j;
-            const bool in = gi < K && gj <= gi && !(unit && gi == gj);
-            const int  ci = in ? gi : 0, cj = in ? gj : 0;
-            const int  mi = rev ? K - 1 - ci : ci, mj = rev ? K - 1 - cj : cj;
-            const T    v  = gload(Ap + (tA ? int64_t(mj) + int64_t(mi) * lda
-                                          : int64_t(mi) + int64_t(mj) * lda));
-            reg[s] = in ? v : T(0);
+            const int ci = min(I * PNB + i, K - 1), cj = min(J * PNB + j, K - 1);
+            const int mi = rev ? K - 1 - ci : ci, mj = rev ? K - 1 - cj : cj;
+            reg[s] = gload(Ap + (tA ? int64_t(mj) + int64_t(mi) * lda
+                                    : int64_t(mi) + int64_t(mj) * lda));
         }
     };
     auto store_tile = [&]() {
@@ -159,7 +161,9 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
         {
             int i, j;
             tile_ij(s, i, j);
-            Ls[i * PLDA + j] = reg[s];
+            const int  gi = tI * PNB + i, gj = tJ * PNB + j;
+            const bool in = gi < K && gj <= gi && !(unit && gi == gj);
+            Ls[i * PLDA + j] = in ? reg[s] : T(0);
         }
     };
 
@@ -167,8 +171,16 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
     load_tile(0, 0);
     for(int I = 0; I < nblk; ++I)
     {
-        // ---- c_I accumulation: sum_{J<I} L_IJ y_J on MFMA ----
-        typename Mfma<T>::acc_t acc{0, 0, 0, 0};
+        // right-hand sides of this block, consumed after the accumulation
+        T bv[4];
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+        {
+            const int gi = I * PNB + g + 16 * r;
+            bv[r]        = gload(Bp + ((vok && gi < K) ? boff(gi, vec) : 0));
+        }
+        // ---- c_I accumulation: sum_{J<I} L_IJ y_J on MFMA (two chains) ----
+        typename Mfma<T>::acc_t acc0{0, 0, 0, 0}, acc1{0, 0, 0, 0};
         for(int J = 0; J < I; ++J)
         {
             __syncthreads(); // previous readers of Ls are done
@@ -176,12 +188,16 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
             __syncthreads();
             load_tile(I, J + 1); // next: (I, J+1), the diagonal block when J+1 == I
 #pragma unroll
-            for(int k4 = 0; k4 < PNB / 4; ++k4)
+            for(int k4 = 0; k4 < PNB / 4; k4 += 2)
             {
-                const int kk = 4 * k4 + (lane >> 4);
-                const T   av = Ls[(16 * w + (lane & 15)) * PLDA + kk];
-                const T   yv = Ys[(J * PNB + kk) * PLDY + (lane & 15)];
-                acc          = Mfma<T>::run(av, yv, acc);
+                const int kk0 = 4 * k4 + (lane >> 4);
+                const int kk1 = kk0 + 4;
+                const T   a0  = Ls[(16 * w + (lane & 15)) * PLDA + kk0];
+                const T   y0  = Ys[(J * PNB + kk0) * PLDY + (lane & 15)];
+                const T   a1  = Ls[(16 * w + (lane & 15)) * PLDA + kk1];
+                const T   y1  = Ys[(J * PNB + kk1) * PLDY + (lane & 15)];
+                acc0          = Mfma<T>::run(a0, y0, acc0);
+                acc1          = Mfma<T>::run(a1, y1, acc1);
             }
         }
         __syncthreads();
@@ -189,7 +205,7 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
         store_tile();
 #pragma unroll
         for(int r = 0; r < 4; ++r)
-            Cs[(16 * w + Mfma<T>::row(lane, r)) * PNC + (lane & 15)] = acc[r];
+            Cs[(16 * w + Mfma<T>::row(lane, r)) * PNC + (lane & 15)] = acc0[r] + acc1[r];
         __syncthreads();
         if(tid < PNB)
         {
@@ -210,8 +226,7 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
         {
             const int  gi = I * PNB + g + 16 * r;
             const bool ok = vok && gi < K;
-            const T    bv = gload(Bp + (ok ? boff(gi, vec) : 0));
-            c[r]          = ok ? scale * bv - Cs[(g + 16 * r) * PNC + vl] : T(0);
+            c[r]          = ok ? scale * bv[r] - Cs[(g + 16 * r) * PNC + vl] : T(0);
         }
         PanelSolve<T, PNB>::run(c, Ls, rdiag, g);
 #pragma unroll
