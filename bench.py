@@ -118,6 +118,8 @@ def main():
                     help="further workloads: all, none, or a comma list of "
                          "single,large,vbatch,serve")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="do not time the vendor librocblas beside the further workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain device copy of 248,320,000 bytes (the side-L "
                          "launch's algorithmic bytes), so a rocprofv3 --pmc run of this "
@@ -230,7 +232,7 @@ def main():
         import bench_legs as bl
         del data
         torch.cuda.empty_cache()
-        vendor = bl.vendor_lib(torch)
+        vendor = None if args.no_vendor else bl.vendor_lib(torch)
         work = {}
         runs = [("single_512x256", "single", lambda: bl.leg_single(torch, rb, synth, oracle, vendor)),
                 ("large_16384", "large", lambda: bl.leg_large(torch, rb, oracle, vendor)),

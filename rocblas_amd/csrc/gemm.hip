@@ -55,8 +55,12 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
 
     // tile index: spread consecutive tiles over the 8 XCDs' L2s in contiguous
     // runs, then group row-panels so neighbouring workgroups share A panels.
+    // With a triangular operand the work per row panel varies, and contiguous
+    // runs would hand one XCD all the long panels: keep dispatch order there
+    // (round-robin over XCDs, each XCD one row panel of every group).
     const int total = a.tiles_m * a.tiles_n;
     int       bid   = blockIdx.x;
+    if constexpr(!TRI)
     {
         const int q = total / 8, r = total % 8;
         const int x = bid % 8, l = bid / 8;
