@@ -84,6 +84,31 @@ typedef enum rocblas_pointer_mode_
     rocblas_pointer_mode_device = 1
 } rocblas_pointer_mode;
 
+/* rocblas-types.h:165-185 (the real types the _ex entry points accept here;
+ * the other values are declared so callers' code compiles) */
+typedef enum rocblas_datatype_
+{
+    rocblas_datatype_f16_r   = 150,
+    rocblas_datatype_f32_r   = 151,
+    rocblas_datatype_f64_r   = 152,
+    rocblas_datatype_f16_c   = 153,
+    rocblas_datatype_f32_c   = 154,
+    rocblas_datatype_f64_c   = 155,
+    rocblas_datatype_i8_r    = 160,
+    rocblas_datatype_u8_r    = 161,
+    rocblas_datatype_i32_r   = 162,
+    rocblas_datatype_u32_r   = 163,
+    rocblas_datatype_i8_c    = 164,
+    rocblas_datatype_u8_c    = 165,
+    rocblas_datatype_i32_c   = 166,
+    rocblas_datatype_u32_c   = 167,
+    rocblas_datatype_bf16_r  = 168,
+    rocblas_datatype_bf16_c  = 169,
+    rocblas_datatype_f8_r    = 170,
+    rocblas_datatype_bf8_r   = 171,
+    rocblas_datatype_invalid = 255
+} rocblas_datatype;
+
 /* ---- handle and auxiliary ------------------------------------------------ */
 
 /* rocblas-auxiliary.h:40; library/src/rocblas_auxiliary.cpp:188-204.
@@ -309,6 +334,67 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_dtrtri_strided_batched(rocblas_handle 
                                                                  rocblas_int      ldinvA,
                                                                  rocblas_stride   stride_invA,
                                                                  rocblas_int      batch_count);
+
+/* ---- trsm_ex ------------------------------------------------------------
+ * rocblas-functions.h:23223-23237, 23404-23419, 23563-23581; implementation
+ * rocblas_trsm.cpp:28-73 (and the batched / strided-batched .cpp files).
+ * compute_type rocblas_datatype_f32_r or _f64_r selects the element type
+ * (others: rocblas_status_not_implemented; the reference also takes the
+ * complex types).  invA (packed 128x128 diagonal-block inverses) is an
+ * optional hint in the reference; this solve forms no inverses and does not
+ * read it, so results equal rocblas_{s,d}trsm's. */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_trsm_ex(rocblas_handle    handle,
+                                                  rocblas_side      side,
+                                                  rocblas_fill      uplo,
+                                                  rocblas_operation transA,
+                                                  rocblas_diagonal  diag,
+                                                  rocblas_int       m,
+                                                  rocblas_int       n,
+                                                  const void*       alpha,
+                                                  const void*       A,
+                                                  rocblas_int       lda,
+                                                  void*             B,
+                                                  rocblas_int       ldb,
+                                                  const void*       invA,
+                                                  rocblas_int       invA_size,
+                                                  rocblas_datatype  compute_type);
+
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_trsm_batched_ex(rocblas_handle    handle,
+                                                          rocblas_side      side,
+                                                          rocblas_fill      uplo,
+                                                          rocblas_operation transA,
+                                                          rocblas_diagonal  diag,
+                                                          rocblas_int       m,
+                                                          rocblas_int       n,
+                                                          const void*       alpha,
+                                                          const void*       A,
+                                                          rocblas_int       lda,
+                                                          void*             B,
+                                                          rocblas_int       ldb,
+                                                          rocblas_int       batch_count,
+                                                          const void*       invA,
+                                                          rocblas_int       invA_size,
+                                                          rocblas_datatype  compute_type);
+
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_trsm_strided_batched_ex(rocblas_handle    handle,
+                                                                  rocblas_side      side,
+                                                                  rocblas_fill      uplo,
+                                                                  rocblas_operation transA,
+                                                                  rocblas_diagonal  diag,
+                                                                  rocblas_int       m,
+                                                                  rocblas_int       n,
+                                                                  const void*       alpha,
+                                                                  const void*       A,
+                                                                  rocblas_int       lda,
+                                                                  rocblas_stride    stride_A,
+                                                                  void*             B,
+                                                                  rocblas_int       ldb,
+                                                                  rocblas_stride    stride_B,
+                                                                  rocblas_int       batch_count,
+                                                                  const void*       invA,
+                                                                  rocblas_int       invA_size,
+                                                                  rocblas_stride    stride_invA,
+                                                                  rocblas_datatype  compute_type);
 
 #ifdef __cplusplus
 }

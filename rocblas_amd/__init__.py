@@ -120,6 +120,13 @@ SYMBOLS = [
     ("rocblas_dtrtri_batched", _i32, _TRTRI_BATCHED),
     ("rocblas_strtri_strided_batched", _i32, _TRTRI_STRIDED),
     ("rocblas_dtrtri_strided_batched", _i32, _TRTRI_STRIDED),
+    ("rocblas_trsm_ex", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32,
+                               _vp, _i32, _i32]),
+    ("rocblas_trsm_batched_ex", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32,
+                                       _vp, _i32, _i32, _vp, _i32, _i32]),
+    ("rocblas_trsm_strided_batched_ex", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp,
+                                               _i32, ctypes.c_int64, _vp, _i32, ctypes.c_int64,
+                                               _i32, _vp, _i32, ctypes.c_int64, _i32]),
     # include/rocblas_amd_ext.h
     ("rocblas_amd_strsm_vbatched", _i32, _TRSM_VBATCHED),
     ("rocblas_amd_dtrsm_vbatched", _i32, _TRSM_VBATCHED),

@@ -4,7 +4,13 @@
 // (library/src/blas3/rocblas_trsm.hpp:29-82 via
 // library/src/blas3/rocblas_trsm_imp.hpp:54-157 and the batched /
 // strided-batched *_imp.hpp files), then hands a TrsmProblem to the launcher
-// in trsm_kernels.hip.  No exception crosses the ABI.
+// in trsm_driver.hip.  No exception crosses the ABI.
+//
+// rocblas_trsm{,_batched,_strided_batched}_ex (library/src/blas3/
+// rocblas_trsm.cpp:28-73 and the batched / strided-batched .cpp files) take the
+// data type at run time and an optional invA of packed diagonal-block
+// inverses.  The solve here forms no inverses, so a supplied invA is accepted
+// and not read; the result is the same solve as the typed entry points.
 #include "internal.hpp"
 
 #include <new>
@@ -192,7 +198,116 @@ using namespace rbamd;
         RBAMD_TRY_END                                                                           \
     }
 
+#define RBAMD_TRSM_EX_DISPATCH(CALL)                   \
+    switch(compute_type)                                \
+    {                                                   \
+    case rocblas_datatype_f32_r:                        \
+    {                                                   \
+        using T = float;                                \
+        CALL;                                           \
+    }                                                   \
+    case rocblas_datatype_f64_r:                        \
+    {                                                   \
+        using T = double;                               \
+        CALL;                                           \
+    }                                                   \
+    default:                                            \
+        return rocblas_status_not_implemented;          \
+    }
+
 extern "C" {
+rocblas_status rocblas_trsm_ex(rocblas_handle    handle,
+                               rocblas_side      side,
+                               rocblas_fill      uplo,
+                               rocblas_operation transA,
+                               rocblas_diagonal  diag,
+                               rocblas_int       m,
+                               rocblas_int       n,
+                               const void*       alpha,
+                               const void*       A,
+                               rocblas_int       lda,
+                               void*             B,
+                               rocblas_int       ldb,
+                               const void*       invA,
+                               rocblas_int       invA_size,
+                               rocblas_datatype  compute_type)
+{
+    (void)invA;
+    (void)invA_size;
+    RBAMD_TRY_BEGIN
+    RBAMD_TRSM_EX_DISPATCH(
+        return trsm_entry<T>(handle, side, uplo, transA, diag, m, n,
+                             static_cast<const T*>(alpha),
+                             BatchPtr<const T>{nullptr, static_cast<const T*>(A), 0, 0},
+                             A == nullptr, lda, BatchPtr<T>{nullptr, static_cast<T*>(B), 0, 0},
+                             B == nullptr, ldb, 1))
+    RBAMD_TRY_END
+}
+
+rocblas_status rocblas_trsm_batched_ex(rocblas_handle    handle,
+                                       rocblas_side      side,
+                                       rocblas_fill      uplo,
+                                       rocblas_operation transA,
+                                       rocblas_diagonal  diag,
+                                       rocblas_int       m,
+                                       rocblas_int       n,
+                                       const void*       alpha,
+                                       const void*       A,
+                                       rocblas_int       lda,
+                                       void*             B,
+                                       rocblas_int       ldb,
+                                       rocblas_int       batch_count,
+                                       const void*       invA,
+                                       rocblas_int       invA_size,
+                                       rocblas_datatype  compute_type)
+{
+    (void)invA;
+    (void)invA_size;
+    RBAMD_TRY_BEGIN
+    RBAMD_TRSM_EX_DISPATCH(
+        return trsm_entry<T>(handle, side, uplo, transA, diag, m, n,
+                             static_cast<const T*>(alpha),
+                             BatchPtr<const T>{static_cast<const T* const*>(A), nullptr, 0, 0},
+                             A == nullptr, lda,
+                             BatchPtr<T>{static_cast<T* const*>(B), nullptr, 0, 0},
+                             B == nullptr, ldb, batch_count))
+    RBAMD_TRY_END
+}
+
+rocblas_status rocblas_trsm_strided_batched_ex(rocblas_handle    handle,
+                                               rocblas_side      side,
+                                               rocblas_fill      uplo,
+                                               rocblas_operation transA,
+                                               rocblas_diagonal  diag,
+                                               rocblas_int       m,
+                                               rocblas_int       n,
+                                               const void*       alpha,
+                                               const void*       A,
+                                               rocblas_int       lda,
+                                               rocblas_stride    stride_A,
+                                               void*             B,
+                                               rocblas_int       ldb,
+                                               rocblas_stride    stride_B,
+                                               rocblas_int       batch_count,
+                                               const void*       invA,
+                                               rocblas_int       invA_size,
+                                               rocblas_stride    stride_invA,
+                                               rocblas_datatype  compute_type)
+{
+    (void)invA;
+    (void)invA_size;
+    (void)stride_invA;
+    RBAMD_TRY_BEGIN
+    RBAMD_TRSM_EX_DISPATCH(
+        return trsm_entry<T>(handle, side, uplo, transA, diag, m, n,
+                             static_cast<const T*>(alpha),
+                             BatchPtr<const T>{nullptr, static_cast<const T*>(A), stride_A, 0},
+                             A == nullptr, lda,
+                             BatchPtr<T>{nullptr, static_cast<T*>(B), stride_B, 0}, B == nullptr,
+                             ldb, batch_count))
+    RBAMD_TRY_END
+}
+
 RBAMD_TRSM_PLAIN(rocblas_strsm, float)
 RBAMD_TRSM_PLAIN(rocblas_dtrsm, double)
 RBAMD_TRSM_BATCHED(rocblas_strsm_batched, float)

@@ -403,3 +403,56 @@ def test_large_single_solve_backward_error(h):
                     logical(dB, m, m, n))
     say("large eta", eta)
     assert eta <= 1e-14
+
+
+# ------------------------------------------------------------ trsm_ex --
+
+def test_trsm_ex_forms_match_typed_entry_points(h):
+    """rocblas_trsm{,_batched,_strided_batched}_ex (library/src/blas3/rocblas_trsm.cpp:28-73):
+    f64/f32 by compute_type give the typed entry points' results bit for bit, with or
+    without an invA hint; other types are not implemented; argument checks are
+    trsm's."""
+    rng = np.random.default_rng(77)
+    L = rb.lib()
+    m, n, batch = 200, 37, 3
+    for prec, dt, code in (("d", np.float64, 152), ("s", np.float32, 151)):
+        A = oracle.diag_dominant_triangular(m, "U", rng, poison=0.0).astype(dt)
+        B = rng.uniform(-1, 1, (m, n)).astype(dt)
+        A, B = np.asfortranarray(A), np.asfortranarray(B)
+        say("trsm_ex", prec)
+        dA = dev(A)
+        want = dev(B)
+        assert rb.trsm(h, "L", "U", "T", "N", m, n, 1.5, dA.data_ptr(), m, want.data_ptr(), m,
+                       precision=prec) == S.success
+        alpha = (ctypes.c_double if prec == "d" else ctypes.c_float)(1.5)
+        junk = torch.full((128 * m,), float("nan"), dtype=dA.dtype, device="cuda")
+        for invA, size in ((None, 0), (junk.data_ptr(), 128 * m)):
+            got = dev(B)
+            assert L.rocblas_trsm_ex(h.ptr, 141, 121, 112, 131, m, n, ctypes.addressof(alpha),
+                                     dA.data_ptr(), m, got.data_ptr(), m, invA, size,
+                                     code) == S.success
+            torch.cuda.synchronize()
+            assert torch.equal(got, want)
+        # strided and pointer-array batched
+        dAs = torch.cat([dA] * batch)
+        dBs = torch.cat([dev(B)] * batch)
+        assert L.rocblas_trsm_strided_batched_ex(h.ptr, 141, 121, 112, 131, m, n,
+                                                 ctypes.addressof(alpha), dAs.data_ptr(), m, m * m,
+                                                 dBs.data_ptr(), m, m * n, batch, None, 0, 0,
+                                                 code) == S.success
+        Bl = [dev(B) for _ in range(batch)]
+        pa, pb = ptr_array([dA] * batch), ptr_array(Bl)
+        assert L.rocblas_trsm_batched_ex(h.ptr, 141, 121, 112, 131, m, n, ctypes.addressof(alpha),
+                                         pa.data_ptr(), m, pb.data_ptr(), m, batch, None, 0,
+                                         code) == S.success
+        torch.cuda.synchronize()
+        for b in range(batch):
+            assert torch.equal(dBs[b * m * n:(b + 1) * m * n], want)
+            assert torch.equal(Bl[b], want)
+    one = ctypes.c_double(1.0)
+    assert L.rocblas_trsm_ex(h.ptr, 141, 121, 111, 131, 4, 4, ctypes.addressof(one), None, 4, None,
+                             4, None, 0, 154) == S.not_implemented
+    assert L.rocblas_trsm_ex(h.ptr, 141, 121, 111, 131, 4, 4, ctypes.addressof(one), None, 3, None,
+                             4, None, 0, 152) == S.invalid_size
+    assert L.rocblas_trsm_ex(None, 141, 121, 111, 131, 4, 4, ctypes.addressof(one), None, 4, None,
+                             4, None, 0, 152) == S.invalid_handle
