@@ -335,6 +335,99 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_dtrtri_strided_batched(rocblas_handle 
                                                                  rocblas_stride   stride_invA,
                                                                  rocblas_int      batch_count);
 
+/* ---- ILP64 trsm ---------------------------------------------------------
+ * rocblas-functions.h:19441-19466, 19631-19658, 19839-19870; implementation
+ * library/src/src64/blas3/rocblas_trsm_64.cpp, rocblas_trsm_kernels_64.cpp:31-225.
+ * Same checks on 64-bit sizes; a triangle dimension above INT32_MAX returns
+ * rocblas_status_invalid_size (as the reference); leading dimensions above
+ * INT32_MAX return rocblas_status_not_implemented; other sizes are solved by
+ * the 32-bit path in chunks of right-hand sides and batch entries. */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_strsm_64(rocblas_handle    handle,
+                                                   rocblas_side      side,
+                                                   rocblas_fill      uplo,
+                                                   rocblas_operation transA,
+                                                   rocblas_diagonal  diag,
+                                                   int64_t           m,
+                                                   int64_t           n,
+                                                   const float*      alpha,
+                                                   const float*      A,
+                                                   int64_t           lda,
+                                                   float*            B,
+                                                   int64_t           ldb);
+
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_strsm_batched_64(rocblas_handle    handle,
+                                                           rocblas_side      side,
+                                                           rocblas_fill      uplo,
+                                                           rocblas_operation transA,
+                                                           rocblas_diagonal  diag,
+                                                           int64_t           m,
+                                                           int64_t           n,
+                                                           const float*        alpha,
+                                                           const float* const  A[],
+                                                           int64_t           lda,
+                                                           float* const        B[],
+                                                           int64_t           ldb,
+                                                           int64_t           batch_count);
+
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_strsm_strided_batched_64(rocblas_handle    handle,
+                                                                   rocblas_side      side,
+                                                                   rocblas_fill      uplo,
+                                                                   rocblas_operation transA,
+                                                                   rocblas_diagonal  diag,
+                                                                   int64_t           m,
+                                                                   int64_t           n,
+                                                                   const float*        alpha,
+                                                                   const float*        A,
+                                                                   int64_t           lda,
+                                                                   rocblas_stride    stride_a,
+                                                                   float*              B,
+                                                                   int64_t           ldb,
+                                                                   rocblas_stride    stride_b,
+                                                                   int64_t           batch_count);
+
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_dtrsm_64(rocblas_handle    handle,
+                                                   rocblas_side      side,
+                                                   rocblas_fill      uplo,
+                                                   rocblas_operation transA,
+                                                   rocblas_diagonal  diag,
+                                                   int64_t           m,
+                                                   int64_t           n,
+                                                   const double*     alpha,
+                                                   const double*     A,
+                                                   int64_t           lda,
+                                                   double*           B,
+                                                   int64_t           ldb);
+
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_dtrsm_batched_64(rocblas_handle    handle,
+                                                           rocblas_side      side,
+                                                           rocblas_fill      uplo,
+                                                           rocblas_operation transA,
+                                                           rocblas_diagonal  diag,
+                                                           int64_t           m,
+                                                           int64_t           n,
+                                                           const double*        alpha,
+                                                           const double* const  A[],
+                                                           int64_t           lda,
+                                                           double* const        B[],
+                                                           int64_t           ldb,
+                                                           int64_t           batch_count);
+
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_dtrsm_strided_batched_64(rocblas_handle    handle,
+                                                                   rocblas_side      side,
+                                                                   rocblas_fill      uplo,
+                                                                   rocblas_operation transA,
+                                                                   rocblas_diagonal  diag,
+                                                                   int64_t           m,
+                                                                   int64_t           n,
+                                                                   const double*        alpha,
+                                                                   const double*        A,
+                                                                   int64_t           lda,
+                                                                   rocblas_stride    stride_a,
+                                                                   double*              B,
+                                                                   int64_t           ldb,
+                                                                   rocblas_stride    stride_b,
+                                                                   int64_t           batch_count);
+
 /* ---- trsm_ex ------------------------------------------------------------
  * rocblas-functions.h:23223-23237, 23404-23419, 23563-23581; implementation
  * rocblas_trsm.cpp:28-73 (and the batched / strided-batched .cpp files).

@@ -82,6 +82,7 @@ _lib = None
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
+_i64 = ctypes.c_int64
 
 # (name, restype, argtypes) for every symbol declared in include/rocblas_amd.h
 _TRSM_PLAIN = [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32]
@@ -120,6 +121,16 @@ SYMBOLS = [
     ("rocblas_dtrtri_batched", _i32, _TRTRI_BATCHED),
     ("rocblas_strtri_strided_batched", _i32, _TRTRI_STRIDED),
     ("rocblas_dtrtri_strided_batched", _i32, _TRTRI_STRIDED),
+    ("rocblas_strsm_64", _i32, [_vp, _i32, _i32, _i32, _i32, _i64, _i64, _vp, _vp, _i64, _vp, _i64]),
+    ("rocblas_strsm_batched_64", _i32, [_vp, _i32, _i32, _i32, _i32, _i64, _i64, _vp, _vp, _i64, _vp,
+                                        _i64, _i64]),
+    ("rocblas_strsm_strided_batched_64", _i32, [_vp, _i32, _i32, _i32, _i32, _i64, _i64, _vp, _vp,
+                                                _i64, _i64, _vp, _i64, _i64, _i64]),
+    ("rocblas_dtrsm_64", _i32, [_vp, _i32, _i32, _i32, _i32, _i64, _i64, _vp, _vp, _i64, _vp, _i64]),
+    ("rocblas_dtrsm_batched_64", _i32, [_vp, _i32, _i32, _i32, _i32, _i64, _i64, _vp, _vp, _i64, _vp,
+                                        _i64, _i64]),
+    ("rocblas_dtrsm_strided_batched_64", _i32, [_vp, _i32, _i32, _i32, _i32, _i64, _i64, _vp, _vp,
+                                                _i64, _i64, _vp, _i64, _i64, _i64]),
     ("rocblas_trsm_ex", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32,
                                _vp, _i32, _i32]),
     ("rocblas_trsm_batched_ex", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32,

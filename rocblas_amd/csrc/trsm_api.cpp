@@ -13,6 +13,7 @@
 // and not read; the result is the same solve as the typed entry points.
 #include "internal.hpp"
 
+#include <algorithm>
 #include <new>
 
 namespace rbamd
@@ -111,6 +112,76 @@ rocblas_status trsm_entry(rocblas_handle    handle,
         p.alpha_dev = alpha;
     }
     return trsm_launch<T>(handle, p);
+}
+
+// ILP64 forms (library/src/src64/blas3/rocblas_trsm_64.cpp,
+// rocblas_trsm_kernels_64.cpp:31-225): the same checks on 64-bit sizes, a
+// triangle larger than INT32_MAX is invalid_size as in the reference, and the
+// work is handed to the 32-bit path in chunks of right-hand sides and of the
+// batch.  Leading dimensions beyond INT32_MAX are not implemented here.
+template <typename T>
+rocblas_status trsm64_entry(rocblas_handle    handle,
+                            rocblas_side      side,
+                            rocblas_fill      uplo,
+                            rocblas_operation transA,
+                            rocblas_diagonal  diag,
+                            int64_t           m,
+                            int64_t           n,
+                            const T*          alpha,
+                            BatchPtr<const T> A,
+                            bool              A_null,
+                            int64_t           lda,
+                            BatchPtr<T>       B,
+                            bool              B_null,
+                            int64_t           ldb,
+                            int64_t           batch_count)
+{
+    constexpr int64_t i32max = 0x7fffffff;
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(side != rocblas_side_left && side != rocblas_side_right)
+        return rocblas_status_invalid_value;
+    if(uplo != rocblas_fill_lower && uplo != rocblas_fill_upper)
+        return rocblas_status_invalid_value;
+    if(transA != rocblas_operation_none && transA != rocblas_operation_transpose
+       && transA != rocblas_operation_conjugate_transpose)
+        return rocblas_status_invalid_value;
+    if(diag != rocblas_diagonal_non_unit && diag != rocblas_diagonal_unit)
+        return rocblas_status_invalid_value;
+    const bool    sideL = side == rocblas_side_left;
+    const int64_t k     = sideL ? m : n;
+    if(batch_count < 0 || m < 0 || n < 0 || lda < k || ldb < m)
+        return rocblas_status_invalid_size;
+    if(!m || !n || !batch_count)
+        return handle->size_query ? rocblas_status_size_unchanged : rocblas_status_success;
+    if(k > i32max)
+        return rocblas_status_invalid_size; // exceeds practical memory (reference behaviour)
+    if(lda > i32max || ldb > i32max)
+        return rocblas_status_not_implemented;
+    // chunks: right-hand-side vectors (side L: columns, side R: rows) and batch
+    const int64_t nv    = sideL ? n : m;
+    const int64_t vstep = int64_t(1) << 30;
+    const int64_t bstep = int64_t(1) << 30;
+    for(int64_t b0 = 0; b0 < batch_count; b0 += bstep)
+    {
+        const int64_t bc = std::min(bstep, batch_count - b0);
+        BatchPtr<const T> Ab = A.arr ? BatchPtr<const T>{A.arr + b0, nullptr, 0, A.offset}
+                                     : BatchPtr<const T>{nullptr, A.base, A.stride,
+                                                         A.offset + b0 * A.stride};
+        BatchPtr<T> Bb = B.arr ? BatchPtr<T>{B.arr + b0, nullptr, 0, B.offset}
+                               : BatchPtr<T>{nullptr, B.base, B.stride, B.offset + b0 * B.stride};
+        for(int64_t v0 = 0; v0 < nv; v0 += vstep)
+        {
+            const int64_t vc  = std::min(vstep, nv - v0);
+            BatchPtr<T>   Bc  = Bb.shifted(sideL ? v0 * ldb : v0);
+            rocblas_status st = trsm_entry<T>(
+                handle, side, uplo, transA, diag, sideL ? int(m) : int(vc), sideL ? int(vc) : int(n),
+                alpha, Ab, A_null, int(lda), Bc, B_null, int(ldb), int(bc));
+            if(st != rocblas_status_success)
+                return st;
+        }
+    }
+    return rocblas_status_success;
 }
 
 } // namespace
@@ -307,6 +378,81 @@ rocblas_status rocblas_trsm_strided_batched_ex(rocblas_handle    handle,
                              ldb, batch_count))
     RBAMD_TRY_END
 }
+
+#define RBAMD_TRSM64_PLAIN(NAME, T)                                                              \
+    rocblas_status NAME(rocblas_handle    handle,                                                \
+                        rocblas_side      side,                                                  \
+                        rocblas_fill      uplo,                                                  \
+                        rocblas_operation transA,                                                \
+                        rocblas_diagonal  diag,                                                  \
+                        int64_t           m,                                                     \
+                        int64_t           n,                                                     \
+                        const T*          alpha,                                                 \
+                        const T*          A,                                                     \
+                        int64_t           lda,                                                   \
+                        T*                B,                                                     \
+                        int64_t           ldb)                                                   \
+    {                                                                                            \
+        RBAMD_TRY_BEGIN                                                                          \
+        return trsm64_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                    \
+                               BatchPtr<const T>{nullptr, A, 0, 0}, A == nullptr, lda,           \
+                               BatchPtr<T>{nullptr, B, 0, 0}, B == nullptr, ldb, 1);             \
+        RBAMD_TRY_END                                                                            \
+    }
+
+#define RBAMD_TRSM64_BATCHED(NAME, T)                                                            \
+    rocblas_status NAME(rocblas_handle    handle,                                                \
+                        rocblas_side      side,                                                  \
+                        rocblas_fill      uplo,                                                  \
+                        rocblas_operation transA,                                                \
+                        rocblas_diagonal  diag,                                                  \
+                        int64_t           m,                                                     \
+                        int64_t           n,                                                     \
+                        const T*          alpha,                                                 \
+                        const T* const    A[],                                                   \
+                        int64_t           lda,                                                   \
+                        T* const          B[],                                                   \
+                        int64_t           ldb,                                                   \
+                        int64_t           batch_count)                                           \
+    {                                                                                            \
+        RBAMD_TRY_BEGIN                                                                          \
+        return trsm64_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                    \
+                               BatchPtr<const T>{A, nullptr, 0, 0}, A == nullptr, lda,           \
+                               BatchPtr<T>{B, nullptr, 0, 0}, B == nullptr, ldb, batch_count);   \
+        RBAMD_TRY_END                                                                            \
+    }
+
+#define RBAMD_TRSM64_STRIDED(NAME, T)                                                            \
+    rocblas_status NAME(rocblas_handle    handle,                                                \
+                        rocblas_side      side,                                                  \
+                        rocblas_fill      uplo,                                                  \
+                        rocblas_operation transA,                                                \
+                        rocblas_diagonal  diag,                                                  \
+                        int64_t           m,                                                     \
+                        int64_t           n,                                                     \
+                        const T*          alpha,                                                 \
+                        const T*          A,                                                     \
+                        int64_t           lda,                                                   \
+                        rocblas_stride    stride_a,                                              \
+                        T*                B,                                                     \
+                        int64_t           ldb,                                                   \
+                        rocblas_stride    stride_b,                                              \
+                        int64_t           batch_count)                                           \
+    {                                                                                            \
+        RBAMD_TRY_BEGIN                                                                          \
+        return trsm64_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                    \
+                               BatchPtr<const T>{nullptr, A, stride_a, 0}, A == nullptr, lda,    \
+                               BatchPtr<T>{nullptr, B, stride_b, 0}, B == nullptr, ldb,          \
+                               batch_count);                                                     \
+        RBAMD_TRY_END                                                                            \
+    }
+
+RBAMD_TRSM64_PLAIN(rocblas_strsm_64, float)
+RBAMD_TRSM64_PLAIN(rocblas_dtrsm_64, double)
+RBAMD_TRSM64_BATCHED(rocblas_strsm_batched_64, float)
+RBAMD_TRSM64_BATCHED(rocblas_dtrsm_batched_64, double)
+RBAMD_TRSM64_STRIDED(rocblas_strsm_strided_batched_64, float)
+RBAMD_TRSM64_STRIDED(rocblas_dtrsm_strided_batched_64, double)
 
 RBAMD_TRSM_PLAIN(rocblas_strsm, float)
 RBAMD_TRSM_PLAIN(rocblas_dtrsm, double)

@@ -456,3 +456,48 @@ def test_trsm_ex_forms_match_typed_entry_points(h):
                              4, None, 0, 152) == S.invalid_size
     assert L.rocblas_trsm_ex(None, 141, 121, 111, 131, 4, 4, ctypes.addressof(one), None, 4, None,
                              4, None, 0, 152) == S.invalid_handle
+
+
+# ------------------------------------------------------------- ILP64 --
+
+def test_ilp64_forms(h):
+    """rocblas_{s,d}trsm{,_batched,_strided_batched}_64
+    (library/src/src64/blas3/rocblas_trsm_64.cpp): 64-bit sizes give the 32-bit
+    results bit for bit; a triangle above INT32_MAX is invalid_size, checks are
+    trsm's."""
+    rng = np.random.default_rng(64)
+    L = rb.lib()
+    m, n, batch = 150, 40, 2
+    A = oracle.diag_dominant_triangular(n, "L", rng, poison=0.0)
+    B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    dA = dev(A)
+    want = dev(B)
+    assert rb.trsm(h, "R", "L", "N", "U", m, n, -2.0, dA.data_ptr(), n, want.data_ptr(),
+                   m) == S.success
+    alpha = ctypes.c_double(-2.0)
+    got = dev(B)
+    assert L.rocblas_dtrsm_64(h.ptr, 142, 122, 111, 132, m, n, ctypes.addressof(alpha),
+                              dA.data_ptr(), n, got.data_ptr(), m) == S.success
+    dAs = torch.cat([dA] * batch)
+    dBs = torch.cat([dev(B)] * batch)
+    assert L.rocblas_dtrsm_strided_batched_64(h.ptr, 142, 122, 111, 132, m, n,
+                                              ctypes.addressof(alpha), dAs.data_ptr(), n, n * n,
+                                              dBs.data_ptr(), m, m * n, batch) == S.success
+    Bl = [dev(B) for _ in range(batch)]
+    pa, pb = ptr_array([dA] * batch), ptr_array(Bl)
+    assert L.rocblas_dtrsm_batched_64(h.ptr, 142, 122, 111, 132, m, n, ctypes.addressof(alpha),
+                                      pa.data_ptr(), n, pb.data_ptr(), m, batch) == S.success
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+    for b in range(batch):
+        assert torch.equal(dBs[b * m * n:(b + 1) * m * n], want)
+        assert torch.equal(Bl[b], want)
+    big = 1 << 31
+    assert L.rocblas_dtrsm_64(h.ptr, 141, 122, 111, 131, big, 1, ctypes.addressof(alpha),
+                              dA.data_ptr(), big, got.data_ptr(), big) == S.invalid_size
+    assert L.rocblas_dtrsm_64(h.ptr, 141, 122, 111, 131, 4, 1, ctypes.addressof(alpha),
+                              dA.data_ptr(), 3, got.data_ptr(), 4) == S.invalid_size
+    assert L.rocblas_dtrsm_64(None, 141, 122, 111, 131, 4, 1, ctypes.addressof(alpha),
+                              dA.data_ptr(), 4, got.data_ptr(), 4) == S.invalid_handle
+    assert L.rocblas_dtrsm_64(h.ptr, 141, 122, 111, 131, 0, 1, ctypes.addressof(alpha),
+                              None, 4, None, 4) == S.success
