@@ -194,8 +194,9 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_ssynth_rhs(rocblas_handle handle,
  * recently used when the arena is full) and a synthetic B: requests are
  * admitted FIFO once they have arrived and their memory fits, up to
  * max_inflight at once; those with m <= small_k are solved together in one
- * _vbatched launch per precision, larger ones by rocblas_{s,d}trsm, over the
- * streams in turn, so new work overlaps work in flight.  latency[i] is the
+ * _vbatched launch per precision on the first stream, larger ones by
+ * rocblas_{s,d}trsm over the other streams in turn, so small requests never
+ * wait behind large solves and new work overlaps work in flight.  latency[i] is the
  * host time from arrival to observed completion (-1 if rejected).  No
  * counterpart in the reference. */
 typedef struct rocblas_amd_server_* rocblas_amd_server;

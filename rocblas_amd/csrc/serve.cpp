@@ -13,9 +13,10 @@
 //     are on the device at once;
 //   * launch: admitted requests are generated in place (factor on a miss,
 //     right-hand side always) and solved: those with k <= `small_k` go
-//     together into one rocblas_amd_{s,d}trsm_vbatched launch per precision,
-//     larger ones each into rocblas_{s,d}trsm; launches rotate over the
-//     streams, so work admitted while other work runs overlaps on the device;
+//     together into one rocblas_amd_{s,d}trsm_vbatched launch per precision on
+//     stream 0, larger ones each into rocblas_{s,d}trsm rotating over the
+//     other streams, so small requests never queue behind large solves and
+//     work admitted while other work runs overlaps on the device;
 //   * completion: an event per launch is polled; a request's latency is the
 //     host time from its arrival to the poll that saw its launch complete.
 //     Its right-hand side buffer is freed (or kept for
@@ -243,12 +244,19 @@ bool event_acquire(Server* s, hipEvent_t* e)
     return hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
 }
 
-// Launch one group of admitted requests of one precision on the next stream.
+// Launch one group of admitted requests of one precision.  Batched groups of
+// small solves go to stream 0, large solves rotate over the other streams, so
+// a small request never queues behind a large solve on its stream.
 rocblas_status launch_group(Server* s, const std::vector<int64_t>& group, bool batched, int prec,
                             std::deque<Launch>& inflight)
 {
-    const int      si = s->next_stream;
-    s->next_stream    = (s->next_stream + 1) % int(s->streams.size());
+    const int ns = int(s->streams.size());
+    int       si = 0;
+    if(!batched && ns > 1)
+    {
+        si             = 1 + s->next_stream;
+        s->next_stream = (s->next_stream + 1) % (ns - 1);
+    }
     rocblas_handle h  = s->handles[si];
     Launch         L;
     L.reqs = group;
