@@ -85,7 +85,8 @@ enum SmallFlags : int
     SF_UNIT  = 8,  // unit diagonal: the diagonal of A is never read
     SF_COLS  = 16, // the vectors are columns of B (side left)
     SF_SCALE = 32, // multiply the right-hand side by alpha
-    SF_DPP   = 64  // register-resident triangles with DPP row broadcasts (KMAX <= 32)
+    SF_DPP   = 64, // register-resident triangles with DPP row broadcasts (KMAX <= 32)
+    SF_FULL  = 128 // DPP launch with K == KMAX and every slot holding 16 vectors
 };
 
 template <typename T>

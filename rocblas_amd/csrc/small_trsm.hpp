@@ -313,14 +313,26 @@ __global__ void __launch_bounds__(WAVE) small_trsm_lds_kernel(const SmallArgs<T>
 }
 
 // ---------------------------------------------------------------------------
-// DPP variant (KMAX <= 32, slots of >= 16 vectors): each 16-lane row holds
-// its slot's triangle in registers -- entry t of the pair order in lane
-// t % 16, register t / 16 -- and broadcasts the coefficient each FMA needs
-// with one v_mov_b64_dpp row_newbcast, so the substitution reads no LDS.
-// Lane n also holds the reciprocal diagonal entries n + 16 d.  A slot that
-// spans several rows keeps one copy of the triangle per row (the duplicate
-// loads are the same addresses in the same instruction).
+// DPP variant (KMAX <= 32; slots of 16 vectors, four slots per wavefront):
+// each 16-lane row holds its slot's triangle in registers -- entry t of the
+// pair order in lane t % 16, register t / 16, stored negated -- and every
+// update c[i] -= L[i][j] x_j is a single v_fmac_{f64,f32}_dpp whose first
+// operand is broadcast from the owning lane of the row (row_newbcast), so the
+// substitution reads no LDS and spends no separate broadcast instruction.
+// Lane n also holds the reciprocal diagonal entries n + 16 d.  An instance
+// with more than 16 vectors occupies several slots, each loading the
+// triangle (the duplicate loads are the same addresses in one instruction).
+//
+// Slot bookkeeping is wave-uniform: slot s is unit blockIdx.x * 4 + s, and
+// its instance pointers and first vector are read into scalar registers.
+// Side L loads B coalesced down the columns -- load step s covers
+// 64 / KMAX whole columns of one slot -- so every address is a per-lane
+// constant plus a uniform base.  FULL launches (K == KMAX, every slot holds
+// 16 vectors) carry no masks at all.
 // ---------------------------------------------------------------------------
+constexpr int DPP_VP = 16; // vectors per slot
+constexpr int DPP_G  = 4;  // slots per wavefront
+
 template <int N>
 __device__ __forceinline__ double row_bcast(double x)
 {
@@ -333,23 +345,79 @@ __device__ __forceinline__ float row_bcast(float x)
     return __builtin_amdgcn_mov_dpp(x, 0x150 + N, 0xf, 0xf, true);
 }
 
-template <typename T, int KMAX, bool UPPER, bool TA, int I, int J>
-__device__ __forceinline__ void dpp_update(T (&c)[KMAX], const T (&Lr)[(KMAX * (KMAX + 1) / 2 + 15) / 16], T x)
+// Broadcast of lane N of the row, as ordered inline assembly: the compiler
+// can neither hoist it (the diagonal broadcasts would all be computed up
+// front and held in registers) nor merge it.
+template <int N>
+__device__ __forceinline__ double row_bcast_ordered(double x)
 {
-    constexpr int t = lofs<KMAX, UPPER, TA>(I, J);
-    c[I]            = fma(-row_bcast<t % 16>(Lr[t / 16]), x, c[I]);
+    double r;
+    asm volatile("v_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "=v"(r)
+                 : "v"(x), "n"(N));
+    return r;
 }
 
+template <int N>
+__device__ __forceinline__ float row_bcast_ordered(float x)
+{
+    float r;
+    asm volatile("v_mov_b32_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "=v"(r)
+                 : "v"(x), "n"(N));
+    return r;
+}
+
+// c += (lane N of the row's l) * x in one instruction.  W > 0 pads W wait
+// states first: a DPP instruction reading a VGPR that one of the two
+// preceding VALU instructions wrote needs two (five after an EXEC write), and
+// the compiler does not insert them in front of inline assembly.
+template <int N, int W>
+__device__ __forceinline__ void fmac_bcast(double& c, double l, double x)
+{
+    if constexpr(W > 0)
+        asm volatile("s_nop %4\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf "
+                     "bank_mask:0xf bound_ctrl:1"
+                     : "+v"(c)
+                     : "v"(l), "v"(x), "n"(N), "n"(W - 1));
+    else
+        asm volatile(
+            "v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+            : "+v"(c)
+            : "v"(l), "v"(x), "n"(N));
+}
+
+template <int N, int W>
+__device__ __forceinline__ void fmac_bcast(float& c, float l, float x)
+{
+    if constexpr(W > 0)
+        asm volatile("s_nop %4\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf "
+                     "bank_mask:0xf bound_ctrl:1"
+                     : "+v"(c)
+                     : "v"(l), "v"(x), "n"(N), "n"(W - 1));
+    else
+        asm volatile(
+            "v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+            : "+v"(c)
+            : "v"(l), "v"(x), "n"(N));
+}
+
+template <typename T, int KMAX>
+using DppTri = T[(KMAX * (KMAX + 1) / 2 + 15) / 16];
+
+// column J of the substitution: c[I] -= L[I][J] x for I = J+1 .. KMAX-1
 template <typename T, int KMAX, bool UPPER, bool TA, int J, int I>
 struct DppColumn
 {
-    static __device__ __forceinline__ void run(T (&c)[KMAX],
-                                               const T (&Lr)[(KMAX * (KMAX + 1) / 2 + 15) / 16],
-                                               T x)
+    static __device__ __forceinline__ void run(T (&c)[KMAX], const DppTri<T, KMAX>& Lr, T x)
     {
         if constexpr(I < KMAX)
         {
-            dpp_update<T, KMAX, UPPER, TA, I, J>(c, Lr, x);
+            constexpr int t = lofs<KMAX, UPPER, TA>(I, J);
+            // x was written just before the column's first update; the very
+            // first update of the solve also covers a possible EXEC write
+            constexpr int W = I == J + 1 ? (J == 0 ? 5 : 2) : 0;
+            fmac_bcast<t % 16, W>(c[I], Lr[t / 16], x);
             DppColumn<T, KMAX, UPPER, TA, J, I + 1>::run(c, Lr, x);
         }
     }
@@ -358,59 +426,66 @@ struct DppColumn
 template <typename T, int KMAX, bool UPPER, bool TA, int J>
 struct DppSolve
 {
-    static __device__ __forceinline__ void run(T (&c)[KMAX],
-                                               T (&Lr)[(KMAX * (KMAX + 1) / 2 + 15) / 16],
-                                               const T (&Dr)[(KMAX + 15) / 16], T xprev)
+    static __device__ __forceinline__ void run(T (&c)[KMAX], const DppTri<T, KMAX>& Lr,
+                                               const T (&Dr)[(KMAX + 15) / 16])
     {
         if constexpr(J < KMAX)
         {
-            // Mark the triangle registers as redefined once the previous
-            // column's solution exists (an empty asm: no instruction is
-            // emitted).  This column's broadcasts then cannot be hoisted above
-            // the previous column; otherwise all of them are computed up
-            // front and the registers run out.
-#pragma unroll
-            for(int r = 0; r < (KMAX * (KMAX + 1) / 2 + 15) / 16; ++r)
-                asm volatile("" : "+v"(Lr[r]) : "v"(xprev));
             // lane n holds the reciprocal of the stored diagonal entry n + 16 d,
             // i.e. of real index n + 16 d; canonical J is real index JR
             constexpr int JR = (UPPER != TA) ? KMAX - 1 - J : J;
-            const T       x  = c[J] * row_bcast<JR % 16>(Dr[JR / 16]);
-            c[J]      = x;
+            const T       x  = c[J] * row_bcast_ordered<JR % 16>(Dr[JR / 16]);
+            c[J]             = x;
             DppColumn<T, KMAX, UPPER, TA, J, J + 1>::run(c, Lr, x);
-            DppSolve<T, KMAX, UPPER, TA, J + 1>::run(c, Lr, Dr, x);
+            DppSolve<T, KMAX, UPPER, TA, J + 1>::run(c, Lr, Dr);
         }
     }
 };
 
-template <typename T, int KMAX, bool UPPER, bool TA, bool COLS>
+// wave-uniform copy of a value every lane holds
+template <typename P>
+__device__ __forceinline__ P* uniform_ptr(P* p)
+{
+    const uint64_t v  = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<P*>((uint64_t(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ int uniform_int(int v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+template <typename T, int KMAX, bool UPPER, bool TA, bool COLS, bool FULL>
 __global__ void __launch_bounds__(WAVE) small_trsm_dpp_kernel(const SmallArgs<T> a)
 {
-    static_assert(KMAX <= 32, "register-resident triangles up to 32");
+    static_assert(KMAX <= 32 && WAVE % KMAX == 0, "register-resident triangles up to 32");
     constexpr bool REV = UPPER != TA;
     constexpr int  TRI = KMAX * (KMAX + 1) / 2;
     constexpr int  NR  = (TRI + 15) / 16;
     constexpr int  ND  = (KMAX + 15) / 16;
+    constexpr int  VP  = DPP_VP;
+    constexpr int  G   = DPP_G;
+    // side L: one load step covers SPS whole columns; STEPS steps per slot
+    constexpr int SPS   = WAVE / KMAX;
+    constexpr int STEPS = VP / SPS;
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* const RB = reinterpret_cast<T*>(smem_raw); // side-L staging, 64 x bs_stride
-    __shared__ const T* slotA[WAVE];
-    __shared__ T*       slotB[WAVE];
-    __shared__ int      slotV0[WAVE];
+    __shared__ const T* slotA[G];
+    __shared__ T*       slotB[G];
+    __shared__ int      slotV0[G];
 
-    const int  lane  = threadIdx.x;
-    const int  n     = lane & 15;
-    const int  vpl   = a.vp_log2; // >= 4
-    const int  VP    = 1 << vpl;
-    const int  G     = WAVE >> vpl;
-    const int  g     = lane >> vpl;
-    const int  v     = lane & (VP - 1);
-    const int  K     = a.K;
-    const bool unit  = a.flags & SF_UNIT;
-    const bool scale = a.flags & SF_SCALE;
-    const int  lda   = a.lda;
-    const int  ldb   = a.ldb;
-    const int  bs    = a.bs_stride;
+    const unsigned lane = threadIdx.x;
+    const unsigned n    = lane & 15;
+    const unsigned g    = lane >> 4;
+    const int      K    = FULL ? KMAX : a.K;
+    const bool     unit = a.flags & SF_UNIT;
+    const int      lda  = a.lda;
+    const int      ldb  = a.ldb;
+    const int      bs   = a.bs_stride;
+    const int      nvec = a.nvec;
 
     const T alpha = a.alpha_dev ? *a.alpha_dev : a.alpha_host;
 
@@ -433,10 +508,11 @@ __global__ void __launch_bounds__(WAVE) small_trsm_dpp_kernel(const SmallArgs<T>
     }
     __syncthreads();
 
-    T* const   myB   = slotB[g];
+    // this lane's slot (row g) and vector
     const T*   myA   = slotA[g];
-    const int  vec   = slotV0[g] + v;
-    const bool valid = myB != nullptr && vec < a.nvec;
+    T* const   myB   = slotB[g];
+    const int  vec   = slotV0[g] + int(n);
+    const bool valid = FULL || (myB != nullptr && vec < nvec);
 
     if(alpha == T(0))
     {
@@ -446,94 +522,119 @@ __global__ void __launch_bounds__(WAVE) small_trsm_dpp_kernel(const SmallArgs<T>
         return;
     }
 
-    const T* const safeA = slotA[0];
-    T* const       safeB = slotB[0];
+    // Out-of-range lanes load from a valid address (slot 0 always exists) and
+    // discard the value, so no load sits behind a branch.
+    const T* const safeA = uniform_ptr(slotA[0]);
+    T* const       safeB = uniform_ptr(slotB[0]);
 
-    // ---- 1a. right-hand-side loads ----
+    // ---- 1a. right-hand sides ----
     T c[KMAX];
     T bv[COLS ? KMAX : 1];
+    // side L: lane = (vq, rl) loads row rl of column vq + SPS * s of its step
+    const unsigned rl      = lane % KMAX;
+    const unsigned vq      = lane / KMAX;
+    const int64_t  laneoff = int64_t(rl) + int64_t(vq) * ldb;
     if constexpr(COLS)
     {
 #pragma unroll
         for(int s = 0; s < KMAX; ++s)
         {
-            const int  e  = lane + WAVE * s;
-            const int  r  = e % KMAX;
-            const int  v2 = e / KMAX;
-            T* const   Bp = slotB[v2 >> vpl];
-            const int  vc = slotV0[v2 >> vpl] + (v2 & (VP - 1));
-            const bool ok = Bp && r < K && vc < a.nvec;
-            bv[s]         = gload(ok ? Bp + (r + int64_t(vc) * ldb) : safeB);
-            bv[s]         = ok ? bv[s] : T(0);
-            if(s % 8 == 7)
-                asm volatile("" ::: "memory"); // issue in groups: bounded address registers
+            const int slot = s / STEPS;
+            const int k    = s % STEPS;
+            T* const  sB   = uniform_ptr(slotB[slot]);
+            const int sv   = uniform_int(slotV0[slot]) + SPS * k;
+            const T*  p    = sB + (int64_t(sv) * ldb + laneoff);
+            if constexpr(FULL)
+                bv[s] = gload(p);
+            else
+            {
+                const bool ok = sB != nullptr && int(rl) < K && sv + int(vq) < nvec;
+                const T    x  = gload(ok ? p : safeB);
+                bv[s]         = ok ? x : T(0);
+            }
         }
     }
     else
     {
         const T* rowB = valid ? myB + vec : safeB;
-        int      Kc   = K;
 #pragma unroll
         for(int i = 0; i < KMAX; ++i)
         {
-            if(i % 8 == 0)
-                Kc = fresh(Kc);
-            const int  r  = REV ? KMAX - 1 - i : i;
-            const bool ok = valid && r < Kc;
-            const T    x  = gload(ok ? rowB + int64_t(r) * ldb : safeB);
-            c[i]          = ok ? x : T(0);
+            const int r = REV ? KMAX - 1 - i : i;
+            if constexpr(FULL)
+                c[i] = gload(rowB + int64_t(r) * ldb);
+            else
+            {
+                const bool ok = valid && r < K;
+                const T    x  = gload(ok ? rowB + int64_t(r) * ldb : safeB);
+                c[i]          = ok ? x : T(0);
+            }
         }
     }
 
-    // ---- 1b. this lane's share of the triangle and of the diagonal ----
-    T Lr[NR];
+    // ---- 1b. this lane's share of the triangle (negated) and of the diagonal ----
+    DppTri<T, KMAX> Lr;
+    const T* const  rowA   = FULL || myA ? myA : safeA;
+    const bool      slotok = FULL || myA != nullptr;
 #pragma unroll
     for(int r = 0; r < NR; ++r)
     {
-        const int t = n + 16 * r;
-        int       p, q;
-        pair_decode<KMAX, UPPER>(t < TRI ? t : 0, p, q);
-        const bool ok = myA && t < TRI && p < K && q < K && p != q;
-        const T    x  = gload(ok ? myA + (p + int64_t(q) * lda) : safeA);
-        Lr[r]         = ok ? x : T(0);
-        if(r % 8 == 7)
-            asm volatile("" ::: "memory");
+        const unsigned t = n + 16 * r;
+        int            p, q;
+        pair_decode<KMAX, UPPER>(int(t < TRI ? t : 0), p, q);
+        bool ok = p != q;
+        if(TRI % 16 != 0 && r == NR - 1)
+            ok = ok && t < TRI;
+        if constexpr(!FULL)
+            ok = ok && slotok && p < K && q < K;
+        const T x = gload(ok ? rowA + (p + int64_t(q) * lda) : rowA);
+        Lr[r]     = ok ? -x : T(0);
     }
     T Dr[ND];
 #pragma unroll
     for(int d = 0; d < ND; ++d)
     {
         const int  j  = n + 16 * d;
-        const bool ok = myA && j < K && !unit;
-        const T    x  = gload(ok ? myA + (j + int64_t(j) * lda) : safeA);
+        const bool ok = slotok && j < K && !unit;
+        const T    x  = gload(rowA + (ok ? j + int64_t(j) * lda : 0));
         Dr[d]         = ok ? T(1) / x : T(1);
     }
 
     // ---- 1c. side L: stage the columns so lane v can read column v ----
     if constexpr(COLS)
     {
+        const unsigned rr = REV ? KMAX - 1 - rl : rl;
 #pragma unroll
         for(int s = 0; s < KMAX; ++s)
-        {
-            const int e  = lane + WAVE * s;
-            const int r  = e % KMAX;
-            const int v2 = e / KMAX;
-            RB[v2 * bs + (REV ? KMAX - 1 - r : r)] = bv[s];
-        }
+            RB[(vq + SPS * s) * bs + rr] = bv[s];
         __syncthreads();
 #pragma unroll
         for(int i = 0; i < KMAX; ++i)
             c[i] = RB[lane * bs + i];
     }
-    if(scale)
+    if(a.flags & SF_SCALE)
     {
 #pragma unroll
         for(int i = 0; i < KMAX; ++i)
             c[i] *= alpha;
     }
 
-    // ---- 2. forward substitution, coefficients broadcast within the row ----
-    DppSolve<T, KMAX, UPPER, TA, 0>::run(c, Lr, Dr, c[0]);
+    // ---- 2. forward substitution, one DPP multiply-add per update ----
+    // Every operand the DPP instructions read is final here (an empty asm
+    // that redefines it), so the compiler cannot sink the last VALU
+    // instruction producing one of them (a negation, a select, the alpha
+    // scaling) in front of its first DPP reader, where the required wait
+    // states would be missing.  The first update of the solve pads five.
+#pragma unroll
+    for(int r = 0; r < NR; ++r)
+        asm volatile("" : "+v"(Lr[r]));
+#pragma unroll
+    for(int d = 0; d < ND; ++d)
+        asm volatile("" : "+v"(Dr[d]));
+#pragma unroll
+    for(int i = 0; i < KMAX; ++i)
+        asm volatile("" : "+v"(c[i]));
+    DppSolve<T, KMAX, UPPER, TA, 0>::run(c, Lr, Dr);
 
     // ---- 3. store ----
     if constexpr(COLS)
@@ -542,35 +643,44 @@ __global__ void __launch_bounds__(WAVE) small_trsm_dpp_kernel(const SmallArgs<T>
         for(int i = 0; i < KMAX; ++i)
             RB[lane * bs + i] = c[i];
         __syncthreads();
-        const int ln = fresh_lane(lane);
-        int       Kc = fresh(K);
+        const unsigned rr = REV ? KMAX - 1 - rl : rl;
 #pragma unroll
         for(int s = 0; s < KMAX; ++s)
         {
-            if(s % 8 == 0)
-                Kc = fresh(Kc);
-            const int e  = ln + WAVE * s;
-            const int r  = e % KMAX;
-            const int v2 = e / KMAX;
-            T* const  Bp = slotB[v2 >> vpl];
-            const int vc = slotV0[v2 >> vpl] + (v2 & (VP - 1));
-            if(Bp && r < Kc && vc < a.nvec)
-                gstore(Bp + (r + int64_t(vc) * ldb), RB[v2 * bs + (REV ? KMAX - 1 - r : r)]);
+            const int slot = s / STEPS;
+            const int k    = s % STEPS;
+            T* const  sB   = uniform_ptr(slotB[slot]);
+            const int sv   = uniform_int(slotV0[slot]) + SPS * k;
+            const T   y    = RB[(vq + SPS * s) * bs + rr];
+            T* const  p    = sB + (int64_t(sv) * ldb + laneoff);
+            if constexpr(FULL)
+                gstore(p, y);
+            else if(sB != nullptr && int(rl) < K && sv + int(vq) < nvec)
+                gstore(p, y);
         }
     }
     else if(valid)
     {
-        int Kc = K;
+        T* const rowB = myB + vec;
 #pragma unroll
         for(int i = 0; i < KMAX; ++i)
         {
-            if(i % 8 == 0)
-                Kc = fresh(Kc);
             const int r = REV ? KMAX - 1 - i : i;
-            if(r < Kc)
-                gstore(myB + (vec + int64_t(r) * ldb), c[i]);
+            if(FULL || r < K)
+                gstore(rowB + int64_t(r) * ldb, c[i]);
         }
     }
+}
+
+template <typename T, int KMAX, bool UPPER, bool TA, bool COLS>
+static void launch_dpp(rocblas_handle h, const SmallArgs<T>& a, dim3 grid, size_t lds)
+{
+    if(a.flags & SF_FULL)
+        hipLaunchKernelGGL((small_trsm_dpp_kernel<T, KMAX, UPPER, TA, COLS, true>), grid,
+                           dim3(WAVE), lds, h->stream, a);
+    else
+        hipLaunchKernelGGL((small_trsm_dpp_kernel<T, KMAX, UPPER, TA, COLS, false>), grid,
+                           dim3(WAVE), lds, h->stream, a);
 }
 
 template <typename T, int KMAX, bool UPPER, bool TA>
@@ -583,11 +693,9 @@ static rocblas_status launch_small_side(rocblas_handle h, const SmallArgs<T>& a,
         if(a.flags & SF_DPP)
         {
             if(cols)
-                hipLaunchKernelGGL((small_trsm_dpp_kernel<T, KMAX, UPPER, TA, true>), grid,
-                                   dim3(WAVE), lds, h->stream, a);
+                launch_dpp<T, KMAX, UPPER, TA, true>(h, a, grid, lds);
             else
-                hipLaunchKernelGGL((small_trsm_dpp_kernel<T, KMAX, UPPER, TA, false>), grid,
-                                   dim3(WAVE), lds, h->stream, a);
+                launch_dpp<T, KMAX, UPPER, TA, false>(h, a, grid, lds);
             return launch_status();
         }
     }

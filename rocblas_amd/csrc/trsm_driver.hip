@@ -155,9 +155,9 @@ struct SmallPlan
 };
 
 // Two kernel variants (small_trsm.hpp):
-//   DPP  (KMAX <= 32 and more than 8 vectors per instance): slots of
-//        VP >= 16 vectors, triangles in registers; LDS only stages side-L
-//        right-hand sides.
+//   DPP  (KMAX <= 32 and more than 8 vectors per instance): four slots of
+//        16 vectors per wavefront, triangles in registers; LDS only stages
+//        side-L right-hand sides.
 //   LDS  (otherwise): VP = next power of two >= nvec, G = 64 / VP slots per
 //        wavefront capped by small_gmax(KMAX) (by widening the slots);
 //        triangles in LDS.
@@ -173,7 +173,7 @@ static SmallPlan plan_small(int K, int nvec, bool cols, bool& dpp)
     while(vp < std::min(nvec, WAVE))
         vp <<= 1;
     if(dpp)
-        vp = std::max(vp, 16);
+        vp = 16;
     int G = WAVE / vp;
     while(!dpp && G > small_gmax(s.kmax))
     {
@@ -206,6 +206,8 @@ static rocblas_status launch_small(rocblas_handle h, const SmallArgs<T>& args_in
     a.units              = int64_t(batch) * a.chunks;
     a.slot_stride        = s.slot_stride;
     a.bs_stride          = s.bs_stride;
+    if(dpp && a.K == s.kmax && a.nvec % VP == 0 && a.units % s.G == 0)
+        a.flags |= SF_FULL;
     const int64_t blocks = (a.units + s.G - 1) / s.G;
     if(blocks > int64_t(0x7fffffff))
         return rocblas_status_invalid_size;
