@@ -37,6 +37,7 @@
 #include <deque>
 #include <map>
 #include <new>
+#include <set>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -45,52 +46,54 @@ namespace
 {
 using clock_type = std::chrono::steady_clock;
 
-// First-fit sub-allocator over one device allocation (offsets, 256-byte units).
+// Best-fit sub-allocator over one device allocation (offsets, 256-byte units).
+// Free blocks are indexed by offset (coalescing on release) and by size, so an
+// allocation and the "does a hole of this size exist" test are O(log n) however
+// fragmented the arena is.
 class Arena
 {
 public:
     explicit Arena(size_t bytes) : size_(bytes)
     {
-        free_[0] = bytes;
+        insert_free(0, bytes);
     }
     bool alloc(size_t bytes, size_t* off)
     {
-        bytes = (bytes + 255) / 256 * 256;
-        for(auto it = free_.begin(); it != free_.end(); ++it)
-            if(it->second >= bytes)
-            {
-                *off                  = it->first;
-                const size_t rest_off = it->first + bytes;
-                const size_t rest     = it->second - bytes;
-                free_.erase(it);
-                if(rest)
-                    free_[rest_off] = rest;
-                used_ += bytes;
-                peak_ = std::max(peak_, used_);
-                return true;
-            }
-        return false;
+        bytes   = round(bytes);
+        auto it = by_size_.lower_bound({bytes, size_t(0)});
+        if(it == by_size_.end())
+            return false;
+        const size_t boff = it->second, bsize = it->first;
+        erase_free(boff, bsize);
+        if(bsize > bytes)
+            insert_free(boff + bytes, bsize - bytes);
+        *off = boff;
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        return true;
     }
     void release(size_t off, size_t bytes)
     {
-        bytes = (bytes + 255) / 256 * 256;
+        bytes = round(bytes);
         used_ -= bytes;
-        auto it = free_.emplace(off, bytes).first;
-        auto nx = std::next(it);
-        if(nx != free_.end() && it->first + it->second == nx->first)
+        auto nx = free_.lower_bound(off);
+        if(nx != free_.end() && off + bytes == nx->first)
         {
-            it->second += nx->second;
-            free_.erase(nx);
+            bytes += nx->second;
+            erase_free(nx->first, nx->second);
         }
-        if(it != free_.begin())
+        auto pv = free_.lower_bound(off);
+        if(pv != free_.begin())
         {
-            auto pv = std::prev(it);
-            if(pv->first + pv->second == it->first)
+            --pv;
+            if(pv->first + pv->second == off)
             {
-                pv->second += it->second;
-                free_.erase(it);
+                off = pv->first;
+                bytes += pv->second;
+                erase_free(pv->first, pv->second);
             }
         }
+        insert_free(off, bytes);
     }
     size_t peak() const
     {
@@ -102,10 +105,25 @@ public:
     }
 
 private:
-    size_t                   size_;
-    size_t                   used_ = 0;
-    size_t                   peak_ = 0;
-    std::map<size_t, size_t> free_; // offset -> bytes
+    static size_t round(size_t b)
+    {
+        return (b + 255) / 256 * 256;
+    }
+    void insert_free(size_t off, size_t bytes)
+    {
+        free_[off] = bytes;
+        by_size_.emplace(bytes, off);
+    }
+    void erase_free(size_t off, size_t bytes)
+    {
+        free_.erase(off);
+        by_size_.erase({bytes, off});
+    }
+    size_t                              size_;
+    size_t                              used_ = 0;
+    size_t                              peak_ = 0;
+    std::map<size_t, size_t>            free_;    // offset -> bytes
+    std::set<std::pair<size_t, size_t>> by_size_; // (bytes, offset)
 };
 
 struct Factor
@@ -149,6 +167,9 @@ struct rocblas_amd_server_
     int                         max_inflight = 256;
     int                         small_k      = 512;
     std::unordered_map<int64_t, Factor> factors;
+    // resident factors no admitted request uses, least recently used first:
+    // (last, factor id); the eviction candidates
+    std::set<std::pair<uint64_t, int64_t>> idle;
     uint64_t                    stamp = 0;
     // the current run's requests (copied) and per-request device state
     std::vector<rocblas_amd_request> reqv;
@@ -172,18 +193,33 @@ void* dev_ptr(Server* s, size_t off)
     return s->base + off;
 }
 
+// A factor is pinned from admission to retirement of each request using it;
+// unpinned factors are the eviction candidates.
+void pin(Server* s, int64_t id, Factor& f)
+{
+    if(f.users++ == 0)
+        s->idle.erase({f.last, id});
+}
+
+void unpin(Server* s, int64_t id, Factor& f, bool touch)
+{
+    if(--f.users == 0)
+    {
+        if(touch)
+            f.last = ++s->stamp;
+        s->idle.emplace(f.last, id);
+    }
+}
+
 bool evict_for(Server* s, size_t bytes, size_t* off)
 {
     while(!s->arena->alloc(bytes, off))
     {
         // least recently used factor with no in-flight user
-        auto victim = s->factors.end();
-        for(auto it = s->factors.begin(); it != s->factors.end(); ++it)
-            if(it->second.users == 0
-               && (victim == s->factors.end() || it->second.last < victim->second.last))
-                victim = it;
-        if(victim == s->factors.end())
+        if(s->idle.empty())
             return false;
+        auto victim = s->factors.find(s->idle.begin()->second);
+        s->idle.erase(s->idle.begin());
         s->arena->release(victim->second.off, victim->second.bytes);
         if(victim->second.ready)
             (void)hipEventDestroy(victim->second.ready);
@@ -248,7 +284,7 @@ bool event_acquire(Server* s, hipEvent_t* e)
 // small solves go to stream 0, large solves rotate over the other streams, so
 // a small request never queues behind a large solve on its stream.
 rocblas_status launch_group(Server* s, const std::vector<int64_t>& group, bool batched, int prec,
-                            std::deque<Launch>& inflight)
+                            std::vector<std::deque<Launch>>& inflight)
 {
     const int ns = int(s->streams.size());
     int       si = 0;
@@ -372,7 +408,7 @@ rocblas_status launch_group(Server* s, const std::vector<int64_t>& group, bool b
     if(!event_acquire(s, &L.done) || hipEventRecord(L.done, s->streams[si]) != hipSuccess)
         return rocblas_status_internal_error;
     s->stats.launches++;
-    inflight.push_back(std::move(L));
+    inflight[si].push_back(std::move(L));
     return rocblas_status_success;
 }
 
@@ -382,9 +418,7 @@ void retire(Server* s, Launch& L, double now, double* latency)
     {
         const rocblas_amd_request& r = s->reqs[id];
         latency[id]                  = now - r.arrival;
-        Factor& f                    = s->factors[r.factor_id];
-        f.users--;
-        f.last = ++s->stamp;
+        unpin(s, r.factor_id, s->factors[r.factor_id], true);
         if(r.keep)
             s->kept[id] = 1;
         else
@@ -426,7 +460,9 @@ rocblas_status run(Server* s, const rocblas_amd_request* in, int64_t count, doub
     }
 
     std::deque<int64_t> pending;
-    std::deque<Launch>  inflight;
+    // launches in flight, per stream in launch order (a stream completes its
+    // work in order, so only the oldest launch of each stream is polled)
+    std::vector<std::deque<Launch>> inflight(s->streams.size());
     int64_t             next_arrival = 0, inflight_reqs = 0, done = 0;
     const auto          t0           = clock_type::now();
     rocblas_status      st           = rocblas_status_success;
@@ -439,38 +475,21 @@ rocblas_status run(Server* s, const rocblas_amd_request* in, int64_t count, doub
 
         // completions
         bool progressed = false;
-        while(!inflight.empty())
-        {
-            const hipError_t q = hipEventQuery(inflight.front().done);
-            if(q == hipErrorNotReady)
-                break;
-            if(q != hipSuccess)
-                return rocblas_status_internal_error;
-            const int64_t n = int64_t(inflight.front().reqs.size());
-            retire(s, inflight.front(), seconds_since(t0), latency);
-            inflight.pop_front();
-            inflight_reqs -= n;
-            done += n;
-            progressed = true;
-        }
-        // later launches may finish first (other streams)
-        for(auto it = inflight.begin(); it != inflight.end();)
-        {
-            const hipError_t q = hipEventQuery(it->done);
-            if(q == hipSuccess)
+        for(auto& q : inflight)
+            while(!q.empty())
             {
-                const int64_t n = int64_t(it->reqs.size());
-                retire(s, *it, seconds_since(t0), latency);
-                it = inflight.erase(it);
+                const hipError_t e = hipEventQuery(q.front().done);
+                if(e == hipErrorNotReady)
+                    break;
+                if(e != hipSuccess)
+                    return rocblas_status_internal_error;
+                const int64_t n = int64_t(q.front().reqs.size());
+                retire(s, q.front(), seconds_since(t0), latency);
+                q.pop_front();
                 inflight_reqs -= n;
                 done += n;
                 progressed = true;
             }
-            else if(q == hipErrorNotReady)
-                ++it;
-            else
-                return rocblas_status_internal_error;
-        }
 
         // admission (FIFO) under the arena budget and the in-flight cap
         std::vector<int64_t> small[2], large[2];
@@ -495,7 +514,7 @@ rocblas_status run(Server* s, const rocblas_amd_request* in, int64_t count, doub
             if(hit && (fit->second.m != r.m || fit->second.prec != r.precision))
                 return rocblas_status_invalid_value; // one factor id, one shape
             if(hit)
-                fit->second.users++; // pinned from admission to retirement
+                pin(s, r.factor_id, fit->second); // pinned from admission to retirement
             // nothing in flight and still no room (fragmentation by kept
             // results): reject rather than wait for memory that never frees
             auto stuck = [&]() {
@@ -511,7 +530,7 @@ rocblas_status run(Server* s, const rocblas_amd_request* in, int64_t count, doub
             if(!evict_for(s, bbytes, &boff))
             {
                 if(hit)
-                    fit->second.users--;
+                    unpin(s, r.factor_id, fit->second, false);
                 if(stuck())
                     continue;
                 break; // wait for completions
