@@ -196,16 +196,14 @@ def diag_dominant_triangular(k, uplo, rng, lda=None, dtype=np.float64, poison=np
     lda = k if lda is None else lda
     A = np.full((lda, k), poison, dtype=dtype, order="F")
     vals = rng.integers(1, 11, size=(k, k)).astype(dtype) * rng.choice([-1.0, 1.0], size=(k, k))
-    for j in range(k):
-        for i in range(k):
-            if (uplo == "U" and i < j) or (uplo == "L" and i > j):
-                A[i, j] = vals[i, j]
-    for i in range(k):
-        if uplo == "U":
-            s = np.abs(A[i, i + 1:k]).sum() + np.abs(A[:i, i]).sum()
-        else:
-            s = np.abs(A[i + 1:k, i]).sum() + np.abs(A[i, :i]).sum()
-        A[i, i] = dtype(1) if s == 0 else dtype(s * 1.01)
+    # strict uplo triangle (the entries are small integers, so every sum
+    # below is exact whatever its order)
+    strict = np.triu(np.ones((k, k), dtype=bool), 1) if uplo == "U" else \
+        np.tril(np.ones((k, k), dtype=bool), -1)
+    A[:k, :k] = np.where(strict, vals, A[:k, :k])
+    T = np.where(strict, A[:k, :k], dtype(0))  # in A's precision
+    s = np.abs(T).sum(axis=1) + np.abs(T).sum(axis=0)  # row i and column i of the triangle
+    A[np.arange(k), np.arange(k)] = np.where(s == 0, dtype(1), (s * 1.01).astype(dtype))
     return A
 
 

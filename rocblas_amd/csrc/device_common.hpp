@@ -63,6 +63,67 @@ __device__ __forceinline__ void gstore(T* p, T v)
     *(__attribute__((address_space(1))) T*)(p) = v;
 }
 
+// ---------------------------------------------------------------------------
+// Row broadcasts (row_newbcast: lane N of each 16-lane row to the whole row)
+// as inline assembly, shared by the small solve and the panel substitutions.
+// ---------------------------------------------------------------------------
+// Broadcast of lane N of the row, as ordered inline assembly: the compiler
+// can neither hoist it (the diagonal broadcasts would all be computed up
+// front and held in registers) nor merge it.
+template <int N>
+__device__ __forceinline__ double row_bcast_ordered(double x)
+{
+    double r;
+    asm volatile("v_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "=v"(r)
+                 : "v"(x), "n"(N));
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ float row_bcast_ordered(float x)
+{
+    float r;
+    asm volatile("v_mov_b32_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "=v"(r)
+                 : "v"(x), "n"(N));
+    return r;
+}
+
+// c += (lane N of the row's l) * x in one instruction.  W > 0 pads W wait
+// states first: a DPP instruction reading a VGPR that one of the two
+// preceding VALU instructions wrote needs two (five after an EXEC write), and
+// the compiler does not insert them in front of inline assembly.
+template <int N, int W>
+__device__ __forceinline__ void fmac_bcast(double& c, double l, double x)
+{
+    if constexpr(W > 0)
+        asm volatile("s_nop %4\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf "
+                     "bank_mask:0xf bound_ctrl:1"
+                     : "+v"(c)
+                     : "v"(l), "v"(x), "n"(N), "n"(W - 1));
+    else
+        asm volatile(
+            "v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+            : "+v"(c)
+            : "v"(l), "v"(x), "n"(N));
+}
+
+template <int N, int W>
+__device__ __forceinline__ void fmac_bcast(float& c, float l, float x)
+{
+    if constexpr(W > 0)
+        asm volatile("s_nop %4\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf "
+                     "bank_mask:0xf bound_ctrl:1"
+                     : "+v"(c)
+                     : "v"(l), "v"(x), "n"(N), "n"(W - 1));
+    else
+        asm volatile(
+            "v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+            : "+v"(c)
+            : "v"(l), "v"(x), "n"(N));
+}
+
 inline rocblas_status launch_status()
 {
     hipError_t e = hipGetLastError();
@@ -138,6 +199,18 @@ bool panel_fits(rocblas_handle h, int K);
 
 template <typename T>
 rocblas_status launch_panel(rocblas_handle h, const SmallArgs<T>& a, int batch);
+
+// Chained single-instance solve (chain_trsm.hip): one workgroup per (64-row
+// block, 16-vector panel), blocks chained through per-panel completion
+// counters in the handle's chain_flags buffer (kChainFlagsBytes: one 64-bit
+// counter per panel, then an error word).
+constexpr int kChainMaxPanels = int(kChainFlagsBytes / 8) - 1;
+
+template <typename T>
+bool chain_fits(rocblas_handle h, int K, int nvec, int batch);
+
+template <typename T>
+rocblas_status launch_chain(rocblas_handle h, const SmallArgs<T>& a);
 
 // Launch one small-solve grid; defined per KMAX in small_k*.hip.
 template <typename T, int KMAX>

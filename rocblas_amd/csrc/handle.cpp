@@ -137,6 +137,18 @@ try
         }
         h->ws_size = want;
     }
+    // completion counters of the chained solve, zero between launches
+    if(hipMalloc(&h->chain_flags, rbamd::kChainFlagsBytes) != hipSuccess
+       || hipMemset(h->chain_flags, 0, rbamd::kChainFlagsBytes) != hipSuccess
+       || hipDeviceSynchronize() != hipSuccess)
+    {
+        if(h->chain_flags)
+            (void)hipFree(h->chain_flags);
+        if(h->ws)
+            (void)hipFree(h->ws);
+        delete h;
+        return rocblas_status_memory_error;
+    }
     *handle = h;
     return rocblas_status_success;
 }
@@ -156,6 +168,11 @@ rocblas_status rocblas_destroy_handle(rocblas_handle handle)
         return rocblas_status_invalid_handle;
     handle->ws_in_use = false;
     (void)free_workspace(handle);
+    if(handle->chain_flags)
+    {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(handle->chain_flags);
+    }
     delete handle;
     return rocblas_status_success;
 }

@@ -35,10 +35,16 @@ struct _rocblas_handle
     bool                 ws_in_use     = false;
     bool                 size_query    = false;
     size_t               query_size    = 0;
+    // completion counters of the chained solve (chain_trsm.hip): one per
+    // panel, then a sticky error word; zero between launches
+    void*                chain_flags   = nullptr;
 };
 
 namespace rbamd
 {
+// Bytes of a handle's chain_flags buffer.
+constexpr size_t kChainFlagsBytes = 4096;
+
 // Round a workspace request the way the reference does (64-byte chunks,
 // library/src/include/handle.hpp:56-61).
 inline size_t roundup_ws(size_t s)

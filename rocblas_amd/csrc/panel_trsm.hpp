@@ -70,6 +70,66 @@ struct PanelSolve<T, 0>
     static __device__ __forceinline__ void run(T (&)[4], const T*, const T*, int) {}
 };
 
+// Fused substitution (chained solve): Ln holds -L of the diagonal block with
+// its diagonal and upper part zero, rd[r] = 1 / L_ii of this lane's rows
+// i = g + 16 r.  Step J: the owner lane (g == J % 16) holds
+// x_J = c_J * rd_J in register J / 16, and each update is one
+// v_fmac_{f64,f32}_dpp broadcasting x_J from that lane:
+// c[r] += x_J * (-L[g + 16 r][J]).  Rows <= J are unchanged (their entries of
+// Ln are zero), so after the last step row i still holds c_i and
+// y_i = c_i * rd_i -- the same operations in the same order as PanelSolve.
+//
+// The inline-assembly updates are scheduling barriers, so the LDS reads of
+// column J are issued by hand FUSED_AHEAD steps early, into a ring of
+// FUSED_AHEAD + 1 register quadruples.
+constexpr int FUSED_AHEAD = 4;
+constexpr int FUSED_RING  = FUSED_AHEAD + 1;
+
+template <typename T, int J>
+__device__ __forceinline__ void fused_load(T (&l)[4], const T* Ln, int g)
+{
+#pragma unroll
+    for(int r = 0; r < 4; ++r)
+        l[r] = Ln[(g + 16 * r) * PLDA + J];
+}
+
+template <typename T, int J>
+struct FusedSolve
+{
+    static __device__ __forceinline__ void run(T (&c)[4], const T (&rd)[4], const T* Ln, int g,
+                                               T (&lb)[FUSED_RING][4])
+    {
+        if constexpr(J < PNB)
+        {
+            if constexpr(J + FUSED_AHEAD < PNB)
+                fused_load<T, J + FUSED_AHEAD>(lb[(J + FUSED_AHEAD) % FUSED_RING], Ln, g);
+            constexpr int q  = J / 16;
+            constexpr int jl = J % 16;
+            const T       x  = c[q] * rd[q];
+            T(&l)[4]         = lb[J % FUSED_RING];
+            // x was written just before the step's first update; the solve's
+            // very first update also covers a possible EXEC write
+            fmac_bcast<jl, J == 0 ? 5 : 2>(c[0], x, l[0]);
+            fmac_bcast<jl, 0>(c[1], x, l[1]);
+            fmac_bcast<jl, 0>(c[2], x, l[2]);
+            fmac_bcast<jl, 0>(c[3], x, l[3]);
+            FusedSolve<T, J + 1>::run(c, rd, Ln, g, lb);
+        }
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ void fused_solve(T (&c)[4], const T (&rd)[4], const T* Ln, int g)
+{
+    T lb[FUSED_RING][4];
+    fused_load<T, 0>(lb[0], Ln, g);
+    fused_load<T, 1>(lb[1], Ln, g);
+    fused_load<T, 2>(lb[2], Ln, g);
+    fused_load<T, 3>(lb[3], Ln, g);
+    static_assert(FUSED_AHEAD == 4, "prologue loads FUSED_AHEAD columns");
+    FusedSolve<T, 0>::run(c, rd, Ln, g, lb);
+}
+
 // LDS bytes of one workgroup solving triangles of size up to K
 template <typename T>
 __host__ __device__ constexpr size_t lds_bytes(int K)

@@ -345,63 +345,6 @@ __device__ __forceinline__ float row_bcast(float x)
     return __builtin_amdgcn_mov_dpp(x, 0x150 + N, 0xf, 0xf, true);
 }
 
-// Broadcast of lane N of the row, as ordered inline assembly: the compiler
-// can neither hoist it (the diagonal broadcasts would all be computed up
-// front and held in registers) nor merge it.
-template <int N>
-__device__ __forceinline__ double row_bcast_ordered(double x)
-{
-    double r;
-    asm volatile("v_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
-                 : "=v"(r)
-                 : "v"(x), "n"(N));
-    return r;
-}
-
-template <int N>
-__device__ __forceinline__ float row_bcast_ordered(float x)
-{
-    float r;
-    asm volatile("v_mov_b32_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
-                 : "=v"(r)
-                 : "v"(x), "n"(N));
-    return r;
-}
-
-// c += (lane N of the row's l) * x in one instruction.  W > 0 pads W wait
-// states first: a DPP instruction reading a VGPR that one of the two
-// preceding VALU instructions wrote needs two (five after an EXEC write), and
-// the compiler does not insert them in front of inline assembly.
-template <int N, int W>
-__device__ __forceinline__ void fmac_bcast(double& c, double l, double x)
-{
-    if constexpr(W > 0)
-        asm volatile("s_nop %4\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf "
-                     "bank_mask:0xf bound_ctrl:1"
-                     : "+v"(c)
-                     : "v"(l), "v"(x), "n"(N), "n"(W - 1));
-    else
-        asm volatile(
-            "v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
-            : "+v"(c)
-            : "v"(l), "v"(x), "n"(N));
-}
-
-template <int N, int W>
-__device__ __forceinline__ void fmac_bcast(float& c, float l, float x)
-{
-    if constexpr(W > 0)
-        asm volatile("s_nop %4\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf "
-                     "bank_mask:0xf bound_ctrl:1"
-                     : "+v"(c)
-                     : "v"(l), "v"(x), "n"(N), "n"(W - 1));
-    else
-        asm volatile(
-            "v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
-            : "+v"(c)
-            : "v"(l), "v"(x), "n"(N));
-}
-
 template <typename T, int KMAX>
 using DppTri = T[(KMAX * (KMAX + 1) / 2 + 15) / 16];
 

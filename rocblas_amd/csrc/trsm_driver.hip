@@ -384,9 +384,10 @@ rocblas_status trsm_launch(rocblas_handle h, const TrsmProblem<T>& p)
         return launch_small<T>(h, s, p.batch);
     }
 
-    if(panel_fits<T>(h, K))
+    if(chain_fits<T>(h, K, sideL ? p.n : p.m, p.batch) || panel_fits<T>(h, K))
     {
-        // one launch: the panel kernel scales by alpha itself
+        // one launch that scales by alpha itself: chained blocks for one
+        // instance with few vector panels, else one workgroup per panel
         SmallArgs<T> s{};
         s.A          = p.A;
         s.B          = p.B;
@@ -399,6 +400,8 @@ rocblas_status trsm_launch(rocblas_handle h, const TrsmProblem<T>& p)
                   | SF_SCALE;
         s.alpha_host = p.alpha_host;
         s.alpha_dev  = p.alpha_dev;
+        if(chain_fits<T>(h, K, s.nvec, p.batch))
+            return launch_chain<T>(h, s);
         return launch_panel<T>(h, s, p.batch);
     }
 

@@ -1,13 +1,13 @@
 """Static check of the DPP kernels' compiled code (CPU; needs hipcc).
 
-The small-solve substitution issues its row broadcasts as inline assembly
-(v_fmac_{f64,f32}_dpp, v_mov_{b64,b32}_dpp), for which the compiler inserts no
-hazard padding.  On CDNA a DPP instruction must not read a VGPR written by a
-VALU instruction within the two preceding wait states, nor follow an EXEC
-write within five.  This test compiles every small-solve translation unit
-with the library's own flags to gfx950 assembly and scans each DPP kernel:
-s_nop N counts N+1 wait states, any other instruction one, and a label counts
-as an unknown predecessor.
+The small-solve and chained-solve substitutions issue their row broadcasts as
+inline assembly (v_fmac_{f64,f32}_dpp, v_mov_{b64,b32}_dpp), for which the
+compiler inserts no hazard padding.  On CDNA a DPP instruction must not read a
+VGPR written by a VALU instruction within the two preceding wait states, nor
+follow an EXEC write within five.  This test compiles those translation units
+with the library's own flags to gfx950 assembly and scans each such kernel up
+to its end label: s_nop N counts N+1 wait states, any other instruction one,
+and a label counts as an unknown predecessor.
 """
 import os
 import re
@@ -86,7 +86,7 @@ def kernels(asm, pattern):
             continue
         if cur:
             body.append((i, line))
-            if "s_endpgm" in line:
+            if line.startswith(".Lfunc_end"):
                 if re.search(pattern, cur):
                     yield cur, body
                 cur = None
@@ -102,16 +102,17 @@ def compile_asm(src, out):
 
 @pytest.mark.skipif(not os.path.exists(rbuild.HIPCC), reason="hipcc not available")
 def test_dpp_kernels_have_no_missing_wait_states(tmp_path):
-    srcs = [os.path.join(rbuild.CSRC, f"small_k{k}.hip") for k in (8, 16, 32)]
-    with concurrent.futures.ThreadPoolExecutor(3) as ex:
+    srcs = [os.path.join(rbuild.CSRC, f) for f in
+            ("small_k8.hip", "small_k16.hip", "small_k32.hip", "chain_trsm.hip")]
+    with concurrent.futures.ThreadPoolExecutor(4) as ex:
         outs = list(ex.map(lambda s: compile_asm(s, str(tmp_path / (os.path.basename(s) + ".s"))),
                            srcs))
     checked, problems = 0, []
     for out in outs:
-        for name, body in kernels(open(out).read(), r"small_trsm_dpp_kernel"):
+        for name, body in kernels(open(out).read(), r"small_trsm_dpp_kernel|chain_trsm_kernel"):
             checked += 1
             problems += [f"{name[:70]} line {ln}: {msg}" for ln, msg in scan(body)]
-    assert checked == 3 * 32, checked
+    assert checked == 3 * 32 + 2, checked
     assert not problems, "\n".join(problems[:20])
 
 

@@ -405,6 +405,44 @@ def test_large_single_solve_backward_error(h):
     assert eta <= 1e-14
 
 
+@pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
+def test_chained_single_solve(h, side, uplo, trans):
+    """One instance with a long triangle and a narrow right-hand side runs as
+    the chained launch (a workgroup per 64-row block and 16-vector panel,
+    blocks chained through completion counters): one vector (the trsv case),
+    ragged panels, k not a multiple of 64, unit and non-unit diagonals.  The
+    result meets the oracle at the reference's 40*eps*k bound and equals, bit
+    for bit, the same instance solved inside a batch of two (which takes the
+    one-workgroup-per-panel kernel for k <= 512)."""
+    rng = np.random.default_rng(909)
+    for (k, nv, diag) in [(65, 1, "N"), (130, 17, "U"), (512, 256, "N"), (777, 3, "N"),
+                          (2049, 40, "U"), (4096, 1, "N")]:
+        m, n = (k, nv) if side == "L" else (nv, k)
+        say("chain", side, uplo, trans, diag, m, n)
+        A = oracle.diag_dominant_triangular(k, uplo, rng, lda=k + 1)
+        if diag == "U":
+            oracle.make_unit_diagonal(A, uplo, k)
+        B = np.asfortranarray(rng.uniform(-1, 1, (m + 2, n)))
+        st, got = gpu_trsm(h, side, uplo, trans, diag, m, n, 0.75, A, B)
+        assert st == S.success
+        ref = B.copy(order="F")
+        oracle.trsm(side, uplo, trans, diag, m, n, 0.75, A, ref)
+        assert oracle.norm1_rel_diff(m, n, got, ref) <= 40 * EPS64 * k
+        assert np.array_equal(got[m:], B[m:]), "rows past m must be untouched"
+        if k <= 512:
+            lda, ldb = A.shape[0], B.shape[0]
+            sa, sb = lda * k, ldb * n
+            dA = torch.from_numpy(np.concatenate([A.ravel(order="F")] * 2)).cuda()
+            dB = torch.from_numpy(np.concatenate([B.ravel(order="F")] * 2)).cuda()
+            st = rb.trsm_strided_batched(h, side, uplo, trans, diag, m, n, 0.75, dA.data_ptr(),
+                                         lda, sa, dB.data_ptr(), ldb, sb, 2)
+            assert st == S.success
+            both = dB.cpu().numpy()
+            for b in range(2):
+                assert np.array_equal(np.asfortranarray(both[b * sb:(b + 1) * sb]
+                                                        .reshape(n, ldb).T), got), b
+
+
 # ------------------------------------------------------------ trsm_ex --
 
 def test_trsm_ex_forms_match_typed_entry_points(h):
