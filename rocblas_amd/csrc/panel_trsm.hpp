@@ -9,13 +9,14 @@
 // blocks I, left-looking:
 //
 //   c_I = scale * b_I - sum_{J<I} L_IJ y_J    (FP64/FP32 MFMA, y_J from LDS)
-//   y_I = L_II^{-1} c_I                       (substitution, DPP broadcasts)
+//   y_I = L_II^{-1} c_I                       (substitution, DPP multiply-adds)
 //
 // Four wavefronts: in the update each owns 16 rows of the block (one 16x16
 // MFMA tile over the panel's 16 vectors); in the substitution each owns 4
 // vectors, a 16-lane row per vector, lane g holding rows g, g+16, g+32, g+48,
-// and the solved value of row j is broadcast from lane j%16 with one
-// v_mov_dpp row_newbcast.  Tiles of L are staged through LDS with coalesced
+// and every update of row i by the solved value of row j is one
+// v_fmac_dpp that broadcasts it from lane j%16 (row_newbcast).  Tiles of L
+// are staged through LDS with coalesced
 // loads (either storage order of A), the next tile's loads in flight during
 // the current tile's MFMAs.
 #pragma once
@@ -33,51 +34,14 @@ constexpr int PLDY  = PNC + 1;  // LDS row stride of the solved panel
 constexpr int PNT   = 256;      // threads
 constexpr int PEPT  = PNB * PNB / PNT; // tile elements per thread
 
-template <int N, typename T>
-__device__ __forceinline__ T bcast(T x)
-{
-    return __builtin_amdgcn_mov_dpp(x, 0x150 + N, 0xf, 0xf, true);
-}
-
-// One step j of the substitution (compile-time j for the DPP lane select).
-template <typename T, int J>
-__device__ __forceinline__ void panel_step(T (&c)[4], const T* Ls, const T* rdiag, int g)
-{
-    constexpr int q  = J / 16;
-    constexpr int jl = J % 16;
-    // the owner lane (g == jl) computes x; every lane of the row receives it
-    const T xo = c[q] * rdiag[J];
-    const T x  = bcast<jl>(xo);
-    if(g == jl)
-        c[q] = x;
-#pragma unroll
-    for(int r = 0; r < 4; ++r)
-        c[r] = fma(-Ls[(g + 16 * r) * PLDA + J], x, c[r]);
-}
-
-template <typename T, int J>
-struct PanelSolve
-{
-    static __device__ __forceinline__ void run(T (&c)[4], const T* Ls, const T* rdiag, int g)
-    {
-        PanelSolve<T, J - 1>::run(c, Ls, rdiag, g);
-        panel_step<T, J - 1>(c, Ls, rdiag, g);
-    }
-};
-template <typename T>
-struct PanelSolve<T, 0>
-{
-    static __device__ __forceinline__ void run(T (&)[4], const T*, const T*, int) {}
-};
-
-// Fused substitution (chained solve): Ln holds -L of the diagonal block with
+// Fused substitution: Ln holds -L of the diagonal block with
 // its diagonal and upper part zero, rd[r] = 1 / L_ii of this lane's rows
 // i = g + 16 r.  Step J: the owner lane (g == J % 16) holds
 // x_J = c_J * rd_J in register J / 16, and each update is one
 // v_fmac_{f64,f32}_dpp broadcasting x_J from that lane:
 // c[r] += x_J * (-L[g + 16 r][J]).  Rows <= J are unchanged (their entries of
 // Ln are zero), so after the last step row i still holds c_i and
-// y_i = c_i * rd_i -- the same operations in the same order as PanelSolve.
+// y_i = c_i * rd_i, which is x_i exactly as the step that owned row i computed it.
 //
 // The inline-assembly updates are scheduling barriers, so the LDS reads of
 // column J are issued by hand FUSED_AHEAD steps early, into a ring of
@@ -226,6 +190,20 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
             Ls[i * PLDA + j] = in ? reg[s] : T(0);
         }
     };
+    // the diagonal block: -L strictly below the diagonal (the fused
+    // substitution's operand), the diagonal into rdiag
+    auto store_diag = [&]() {
+#pragma unroll
+        for(int s = 0; s < PEPT; ++s)
+        {
+            int i, j;
+            tile_ij(s, i, j);
+            const int gi = tI * PNB + i;
+            Ls[i * PLDA + j] = (gi < K && j < i) ? -reg[s] : T(0);
+            if(i == j)
+                rdiag[i] = (gi < K && !unit) ? reg[s] : T(1);
+        }
+    };
 
     // tile sequence: (0,0), (1,0), (1,1), (2,0), ... ; (I, I) is the diagonal block
     load_tile(0, 0);
@@ -261,43 +239,49 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
             }
         }
         __syncthreads();
-        // diagonal block (I, I): reciprocal diagonal out, zero diagonal in
-        store_tile();
+        // diagonal block (I, I): -L below the diagonal, reciprocal diagonal
+        store_diag();
 #pragma unroll
         for(int r = 0; r < 4; ++r)
             Cs[(16 * w + Mfma<T>::row(lane, r)) * PNC + (lane & 15)] = acc0[r] + acc1[r];
         __syncthreads();
         if(tid < PNB)
-        {
-            const int gi = I * PNB + tid;
-            const T   d  = Ls[tid * PLDA + tid];
-            rdiag[tid]   = (gi < K && !unit) ? T(1) / d : T(1);
-            Ls[tid * PLDA + tid] = T(0);
-        }
+            rdiag[tid] = T(1) / rdiag[tid];
         __syncthreads();
         // prefetch the first tile of the next block during the substitution
         if(I + 1 < nblk)
             load_tile(I + 1, 0);
 
         // ---- y_I = L_II^{-1} c_I ----
-        T c[4];
+        T c[4], rd[4];
 #pragma unroll
         for(int r = 0; r < 4; ++r)
         {
             const int  gi = I * PNB + g + 16 * r;
             const bool ok = vok && gi < K;
             c[r]          = ok ? scale * bv[r] - Cs[(g + 16 * r) * PNC + vl] : T(0);
+            rd[r]         = rdiag[g + 16 * r];
         }
-        PanelSolve<T, PNB>::run(c, Ls, rdiag, g);
+        // every operand of the DPP updates is final here (an empty asm that
+        // redefines it), so no VALU instruction producing one of them is
+        // sunk in front of its first DPP reader
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+        {
+            asm volatile("" : "+v"(c[r]));
+            asm volatile("" : "+v"(rd[r]));
+        }
+        fused_solve<T>(c, rd, Ls, g);
 #pragma unroll
         for(int r = 0; r < 4; ++r)
         {
             const int gi = I * PNB + g + 16 * r;
+            const T   y  = c[r] * rd[r];
             if(gi < K)
             {
-                Ys[gi * PLDY + vl] = c[r];
+                Ys[gi * PLDY + vl] = y;
                 if(vok)
-                    gstore(Bp + boff(gi, vec), c[r]);
+                    gstore(Bp + boff(gi, vec), y);
             }
         }
     }

@@ -1,7 +1,7 @@
 """Static check of the DPP kernels' compiled code (CPU; needs hipcc).
 
-The small-solve and chained-solve substitutions issue their row broadcasts as
-inline assembly (v_fmac_{f64,f32}_dpp, v_mov_{b64,b32}_dpp), for which the
+The small-solve and panel substitutions (small, chained, panel and
+variable-size batch kernels) issue their row broadcasts as inline assembly (v_fmac_{f64,f32}_dpp, v_mov_{b64,b32}_dpp), for which the
 compiler inserts no hazard padding.  On CDNA a DPP instruction must not read a
 VGPR written by a VALU instruction within the two preceding wait states, nor
 follow an EXEC write within five.  This test compiles those translation units
@@ -103,16 +103,18 @@ def compile_asm(src, out):
 @pytest.mark.skipif(not os.path.exists(rbuild.HIPCC), reason="hipcc not available")
 def test_dpp_kernels_have_no_missing_wait_states(tmp_path):
     srcs = [os.path.join(rbuild.CSRC, f) for f in
-            ("small_k8.hip", "small_k16.hip", "small_k32.hip", "chain_trsm.hip")]
-    with concurrent.futures.ThreadPoolExecutor(4) as ex:
+            ("small_k8.hip", "small_k16.hip", "small_k32.hip", "chain_trsm.hip", "panel_trsm.hip",
+             "vbatched.hip")]
+    with concurrent.futures.ThreadPoolExecutor(6) as ex:
         outs = list(ex.map(lambda s: compile_asm(s, str(tmp_path / (os.path.basename(s) + ".s"))),
                            srcs))
     checked, problems = 0, []
     for out in outs:
-        for name, body in kernels(open(out).read(), r"small_trsm_dpp_kernel|chain_trsm_kernel"):
+        pattern = r"small_trsm_dpp_kernel|chain_trsm_kernel|panel_trsm_kernel|vb_panel_kernel"
+        for name, body in kernels(open(out).read(), pattern):
             checked += 1
             problems += [f"{name[:70]} line {ln}: {msg}" for ln, msg in scan(body)]
-    assert checked == 3 * 32 + 2, checked
+    assert checked == 3 * 32 + 3 * 2, checked
     assert not problems, "\n".join(problems[:20])
 
 
