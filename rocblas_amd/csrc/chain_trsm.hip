@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(PNT)
         asm volatile("" : "+v"(c[r]));
         asm volatile("" : "+v"(rd[r]));
     }
-    fused_solve<T>(c, rd, Ln, g);
+    fused_solve<T>(c, rd, Ln, g, min(PNB, K - I * PNB));
 #pragma unroll
     for(int r = 0; r < 4; ++r)
     {

@@ -61,10 +61,14 @@ template <typename T, int J>
 struct FusedSolve
 {
     static __device__ __forceinline__ void run(T (&c)[4], const T (&rd)[4], const T* Ln, int g,
-                                               T (&lb)[FUSED_RING][4])
+                                               int nrows, T (&lb)[FUSED_RING][4])
     {
         if constexpr(J < PNB)
         {
+            // rows past the block's last real row hold c = 0: their steps
+            // would add exact zeros (uniform branch)
+            if(J >= nrows)
+                return;
             if constexpr(J + FUSED_AHEAD < PNB)
                 fused_load<T, J + FUSED_AHEAD>(lb[(J + FUSED_AHEAD) % FUSED_RING], Ln, g);
             constexpr int q  = J / 16;
@@ -77,13 +81,15 @@ struct FusedSolve
             fmac_bcast<jl, 0>(c[1], x, l[1]);
             fmac_bcast<jl, 0>(c[2], x, l[2]);
             fmac_bcast<jl, 0>(c[3], x, l[3]);
-            FusedSolve<T, J + 1>::run(c, rd, Ln, g, lb);
+            FusedSolve<T, J + 1>::run(c, rd, Ln, g, nrows, lb);
         }
     }
 };
 
+// nrows: real rows of the block (steps nrows..63 are skipped)
 template <typename T>
-__device__ __forceinline__ void fused_solve(T (&c)[4], const T (&rd)[4], const T* Ln, int g)
+__device__ __forceinline__ void fused_solve(T (&c)[4], const T (&rd)[4], const T* Ln, int g,
+                                            int nrows)
 {
     T lb[FUSED_RING][4];
     fused_load<T, 0>(lb[0], Ln, g);
@@ -91,7 +97,7 @@ __device__ __forceinline__ void fused_solve(T (&c)[4], const T (&rd)[4], const T
     fused_load<T, 2>(lb[2], Ln, g);
     fused_load<T, 3>(lb[3], Ln, g);
     static_assert(FUSED_AHEAD == 4, "prologue loads FUSED_AHEAD columns");
-    FusedSolve<T, 0>::run(c, rd, Ln, g, lb);
+    FusedSolve<T, 0>::run(c, rd, Ln, g, nrows, lb);
 }
 
 // LDS bytes of one workgroup solving triangles of size up to K
@@ -271,7 +277,7 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
             asm volatile("" : "+v"(c[r]));
             asm volatile("" : "+v"(rd[r]));
         }
-        fused_solve<T>(c, rd, Ls, g);
+        fused_solve<T>(c, rd, Ls, g, min(PNB, K - I * PNB));
 #pragma unroll
         for(int r = 0; r < 4; ++r)
         {
