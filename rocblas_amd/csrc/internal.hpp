@@ -137,6 +137,21 @@ struct TrsmProblem
 template <typename T>
 rocblas_status trsm_launch(rocblas_handle h, const TrsmProblem<T>& p);
 
+// invA := op(A)^-1 for batch_count n x n triangles (no argument checks).  With
+// scratch (n * n * batch_count elements) A is copied there first, which makes
+// invA == A (in place) safe.
+template <typename T>
+rocblas_status trtri_core(rocblas_handle    handle,
+                          rocblas_fill      uplo,
+                          rocblas_diagonal  diag,
+                          int               n,
+                          BatchPtr<const T> A,
+                          int               lda,
+                          BatchPtr<T>       invA,
+                          int               ldinvA,
+                          int               batch_count,
+                          T*                scratch);
+
 template <typename T>
 rocblas_status zero_matrix_launch(rocblas_handle h, int m, int n, BatchPtr<T> B, int ldb,
                                   int batch);

@@ -56,14 +56,32 @@ rocblas_status trtri_entry(rocblas_handle    handle,
     if(A_null || invA_null)
         return rocblas_status_invalid_pointer;
 
-    WorkspaceLease    lease(handle, ws_bytes);
+    WorkspaceLease lease(handle, ws_bytes);
+    if(in_place && !lease.ptr)
+        return rocblas_status_memory_error;
+    return trtri_core<T>(handle, uplo, diag, n, A, lda, invA, ldinvA, batch_count,
+                         in_place ? static_cast<T*>(lease.ptr) : nullptr);
+}
+
+} // namespace
+
+template <typename T>
+rocblas_status trtri_core(rocblas_handle    handle,
+                          rocblas_fill      uplo,
+                          rocblas_diagonal  diag,
+                          int               n,
+                          BatchPtr<const T> A,
+                          int               lda,
+                          BatchPtr<T>       invA,
+                          int               ldinvA,
+                          int               batch_count,
+                          T*                scratch)
+{
     BatchPtr<const T> Asrc = A;
     int               ldas = lda;
-    if(in_place)
+    if(scratch)
     {
-        if(!lease.ptr)
-            return rocblas_status_memory_error;
-        BatchPtr<T> W{nullptr, static_cast<T*>(lease.ptr), int64_t(n) * n, 0};
+        BatchPtr<T> W{nullptr, scratch, int64_t(n) * n, 0};
         rocblas_status st = copy_launch<T>(handle, n, n, A, lda, W, n, batch_count);
         if(st != rocblas_status_success)
             return st;
@@ -92,7 +110,13 @@ rocblas_status trtri_entry(rocblas_handle    handle,
     return trsm_launch<T>(handle, p);
 }
 
-} // namespace
+template rocblas_status trtri_core<float>(rocblas_handle, rocblas_fill, rocblas_diagonal, int,
+                                          BatchPtr<const float>, int, BatchPtr<float>, int, int,
+                                          float*);
+template rocblas_status trtri_core<double>(rocblas_handle, rocblas_fill, rocblas_diagonal, int,
+                                           BatchPtr<const double>, int, BatchPtr<double>, int,
+                                           int, double*);
+
 } // namespace rbamd
 
 using namespace rbamd;

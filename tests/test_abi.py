@@ -43,8 +43,55 @@ def test_library_exports_every_declared_symbol():
     assert sorted(n for n, _, _ in rb.SYMBOLS) == declared_symbols()
     out = subprocess.run(["nm", "-D", "--defined-only", rb.LIB_PATH], capture_output=True,
                          text=True, check=True).stdout
-    exported = sorted(l.split()[-1] for l in out.splitlines() if " T " in l)
+    exported = sorted(l.split()[-1] for l in out.splitlines()
+                      if " T " in l and not l.split()[-1].startswith("_Z"))
     assert exported == declared_symbols()
+
+
+INTERNAL = [
+    "rocblas_internal_trsm_workspace_size<{T}>(rocblas_side_, rocblas_operation_, int, int, int, "
+    "int, unsigned long*, unsigned long*, unsigned long*, unsigned long*, unsigned long*)",
+    "rocblas_internal_trsm_batched_workspace_size<{T}>(rocblas_side_, rocblas_operation_, int, "
+    "int, int, int, unsigned long*, unsigned long*, unsigned long*, unsigned long*, "
+    "unsigned long*)",
+    "rocblas_internal_trsm_template<{T}>(_rocblas_handle*, rocblas_side_, rocblas_fill_, "
+    "rocblas_operation_, rocblas_diagonal_, int, int, {T} const*, {T} const*, long, int, long, "
+    "{T}*, long, int, long, int, bool, void*, void*, void*, void*, {T} const*, int, long, long)",
+    "rocblas_internal_trsm_batched_template<{T}>(_rocblas_handle*, rocblas_side_, rocblas_fill_, "
+    "rocblas_operation_, rocblas_diagonal_, int, int, {T} const*, {T} const* const*, long, int, "
+    "long, {T}* const*, long, int, long, int, bool, void*, void*, void*, void*, "
+    "{T} const* const*, int, long, long)",
+    "rocblas_internal_trtri_template<{T}>(_rocblas_handle*, rocblas_fill_, rocblas_diagonal_, "
+    "int, {T} const*, long, int, long, long, {T}*, long, int, long, long, int, int, {T}*)",
+    "rocblas_internal_trtri_batched_template<{T}>(_rocblas_handle*, rocblas_fill_, "
+    "rocblas_diagonal_, int, {T} const* const*, long, int, long, long, {T}* const*, long, int, "
+    "long, long, int, int, {T}* const*)",
+]
+
+
+def test_internal_cxx_exports():
+    """The rocSOLVER-facing templates (include/rocblas_amd_internal.hpp) are
+    exported for float and double with the reference's signatures
+    (library/src/blas3/rocblas_trsm.hpp:173-257, rocblas_trtri.hpp:1098-1150),
+    so their mangled names are the reference's."""
+    out = subprocess.run(["nm", "-DC", "--defined-only", rb.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    demangled = {l.split(" ", 2)[2] for l in out.splitlines() if "rocblas_internal" in l}
+    want = {"rocblas_status_ " + sig.format(T=t) for sig in INTERNAL for t in ("float", "double")}
+    want.add("rocblas_internal_trtri_temp_elements(int, int)")
+    assert demangled == want, sorted(demangled ^ want)
+
+
+def test_internal_trtri_temp_elements():
+    out = subprocess.run(["nm", "-D", "--defined-only", rb.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    name = [l.split()[-1] for l in out.splitlines() if "rocblas_internal_trtri_temp_elements"
+            in l][0]
+    fn = getattr(rb.lib(), name)
+    fn.restype = ctypes.c_size_t
+    fn.argtypes = [ctypes.c_int, ctypes.c_int]
+    assert fn(100, 3) == 100 * 100 * 3
+    assert fn(0, 3) == 0 and fn(5, 0) == 0
 
 
 def test_header_compiles_as_c():
