@@ -8,6 +8,9 @@
  *       rocblas_trsm_kernels.cpp:115-140)
  *   rocblas_internal_trsm{,_batched}_template<T>
  *       library/src/blas3/rocblas_trsm.hpp:201-257 (rocblas_trsm_kernels.cpp:28-113)
+ *   rocblas_internal_trsm{,_batched}_{workspace_size,workspace_max_size,template}_64<T>
+ *       library/src/src64/blas3/rocblas_trsm_64.hpp:72-185 (ILP64 sizes; the
+ *       launcher rocblas_trsm_kernels_64.cpp:31-225, max size :433-530)
  *   rocblas_internal_trtri{,_batched}_template<T>, rocblas_internal_trtri_temp_elements
  *       library/src/blas3/rocblas_trtri.hpp:1098-1150 (rocblas_trtri_kernels.cpp:89-120)
  *
@@ -19,7 +22,10 @@
  * paths) and the trsm workspace / invA arguments are accepted and not read.
  * The trtri templates use w_C_tmp only when invA is A itself (in place): it
  * must then hold rocblas_internal_trtri_temp_elements(n, batch_count)
- * elements, n * n per matrix solved at a time.
+ * elements, n * n per matrix solved at a time.  The _64 forms take 64-bit
+ * sizes: a triangle beyond INT32_MAX is rocblas_status_invalid_size (as in the
+ * reference), leading dimensions beyond INT32_MAX rocblas_status_not_implemented;
+ * their max-size queries report zero bytes with rocblas_status_success.
  */
 #ifndef ROCBLAS_AMD_INTERNAL_HPP
 #define ROCBLAS_AMD_INTERNAL_HPP
@@ -27,6 +33,7 @@
 #include "rocblas_amd.h"
 
 #include <cstddef>
+#include <cstdint>
 
 template <typename T>
 ROCBLAS_AMD_EXPORT rocblas_status
@@ -113,6 +120,119 @@ ROCBLAS_AMD_EXPORT rocblas_status
                                            rocblas_int       supplied_invA_size = 0,
                                            rocblas_stride    offset_invA        = 0,
                                            rocblas_stride    stride_invA        = 0);
+
+template <typename T>
+ROCBLAS_AMD_EXPORT rocblas_status
+    rocblas_internal_trsm_workspace_size_64(rocblas_side      side,
+                                            rocblas_operation transA,
+                                            int64_t           m,
+                                            int64_t           n,
+                                            int64_t           lda,
+                                            int64_t           ldb,
+                                            int64_t           batch_count,
+                                            int64_t           supplied_invA_size,
+                                            size_t*           w_x_tmp_size,
+                                            size_t*           w_x_tmp_arr_size,
+                                            size_t*           w_invA_size,
+                                            size_t*           w_invA_arr_size,
+                                            size_t*           w_x_tmp_size_backup);
+
+template <typename T>
+ROCBLAS_AMD_EXPORT rocblas_status
+    rocblas_internal_trsm_batched_workspace_size_64(rocblas_side      side,
+                                                    rocblas_operation transA,
+                                                    int64_t           m,
+                                                    int64_t           n,
+                                                    int64_t           lda,
+                                                    int64_t           ldb,
+                                                    int64_t           batch_count,
+                                                    int64_t           supplied_invA_size,
+                                                    size_t*           w_x_tmp_size,
+                                                    size_t*           w_x_tmp_arr_size,
+                                                    size_t*           w_invA_size,
+                                                    size_t*           w_invA_arr_size,
+                                                    size_t*           w_x_tmp_size_backup);
+
+/* the most any call with m <= M and n <= N needs (rocblas_trsm_kernels_64.cpp:433-530) */
+template <typename T>
+ROCBLAS_AMD_EXPORT rocblas_status
+    rocblas_internal_trsm_workspace_max_size_64(rocblas_side side,
+                                                int64_t      M,
+                                                int64_t      N,
+                                                int64_t      batch_count,
+                                                size_t*      w_x_tmp_size,
+                                                size_t*      w_invA_size,
+                                                size_t*      w_x_tmp_size_backup);
+
+template <typename T>
+ROCBLAS_AMD_EXPORT rocblas_status
+    rocblas_internal_trsm_batched_workspace_max_size_64(rocblas_side side,
+                                                        int64_t      M,
+                                                        int64_t      N,
+                                                        int64_t      batch_count,
+                                                        size_t*      w_x_tmp_size,
+                                                        size_t*      w_x_tmp_arr_size,
+                                                        size_t*      w_invA_size,
+                                                        size_t*      w_invA_arr_size,
+                                                        size_t*      w_x_tmp_size_backup);
+
+template <typename T>
+ROCBLAS_AMD_EXPORT rocblas_status
+    rocblas_internal_trsm_template_64(rocblas_handle    handle,
+                                      rocblas_side      side,
+                                      rocblas_fill      uplo,
+                                      rocblas_operation transA,
+                                      rocblas_diagonal  diag,
+                                      int64_t           m,
+                                      int64_t           n,
+                                      const T*          alpha,
+                                      const T*          A,
+                                      rocblas_stride    offset_A,
+                                      int64_t           lda,
+                                      rocblas_stride    stride_A,
+                                      T*                B,
+                                      rocblas_stride    offset_B,
+                                      int64_t           ldb,
+                                      rocblas_stride    stride_B,
+                                      int64_t           batch_count,
+                                      bool              optimal_mem,
+                                      void*             w_x_temp,
+                                      void*             w_x_temparr,
+                                      void*             invA               = nullptr,
+                                      void*             invAarr            = nullptr,
+                                      const T*          supplied_invA      = nullptr,
+                                      int64_t           supplied_invA_size = 0,
+                                      rocblas_stride    offset_invA        = 0,
+                                      rocblas_stride    stride_invA        = 0);
+
+template <typename T>
+ROCBLAS_AMD_EXPORT rocblas_status
+    rocblas_internal_trsm_batched_template_64(rocblas_handle    handle,
+                                              rocblas_side      side,
+                                              rocblas_fill      uplo,
+                                              rocblas_operation transA,
+                                              rocblas_diagonal  diag,
+                                              int64_t           m,
+                                              int64_t           n,
+                                              const T*          alpha,
+                                              const T* const*   A,
+                                              rocblas_stride    offset_A,
+                                              int64_t           lda,
+                                              rocblas_stride    stride_A,
+                                              T* const*         B,
+                                              rocblas_stride    offset_B,
+                                              int64_t           ldb,
+                                              rocblas_stride    stride_B,
+                                              int64_t           batch_count,
+                                              bool              optimal_mem,
+                                              void*             w_x_temp,
+                                              void*             w_x_temparr,
+                                              void*             invA               = nullptr,
+                                              void*             invAarr            = nullptr,
+                                              const T* const*   supplied_invA      = nullptr,
+                                              int64_t           supplied_invA_size = 0,
+                                              rocblas_stride    offset_invA        = 0,
+                                              rocblas_stride    stride_invA        = 0);
 
 /* batch_count instances (stride_A apart) of sub_batch_count matrices each
  * (sub_stride_A apart); rocblas_trtri.hpp:1102-1119 */

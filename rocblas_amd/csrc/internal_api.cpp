@@ -1,6 +1,7 @@
 // internal_api.cpp -- the C++ template entry points rocSOLVER links against
 // (include/rocblas_amd_internal.hpp): rocblas_internal_trsm{,_batched}_template,
-// their workspace queries and rocblas_internal_trtri{,_batched}_template.
+// their workspace queries, their ILP64 _64 forms and
+// rocblas_internal_trtri{,_batched}_template.
 //
 // As in the reference (library/src/blas3/rocblas_trsm_kernels.cpp:28-140,
 // rocblas_trsm_kernels.hpp:3594-3626) these check no arguments beyond the
@@ -10,6 +11,9 @@
 // strides address each operand as base + offset + b * stride (or
 // array[b] + offset for the pointer-array forms).
 #include "internal.hpp"
+
+#include <algorithm>
+#include <cstdint>
 
 #include "../../include/rocblas_amd_internal.hpp"
 
@@ -62,6 +66,59 @@ rocblas_status internal_trsm(rocblas_handle    handle,
     else
         p.alpha_dev = alpha;
     return trsm_launch<T>(handle, p);
+}
+
+// ILP64 forms (library/src/src64/blas3/rocblas_trsm_kernels_64.cpp:31-225):
+// a triangle beyond INT32_MAX is invalid_size as in the reference; otherwise
+// the work goes to the 32-bit path in chunks of right-hand-side vectors and of
+// the batch.  Leading dimensions beyond INT32_MAX are not implemented.
+template <typename T>
+rocblas_status internal_trsm_64(rocblas_handle    handle,
+                                rocblas_side      side,
+                                rocblas_fill      uplo,
+                                rocblas_operation transA,
+                                rocblas_diagonal  diag,
+                                int64_t           m,
+                                int64_t           n,
+                                const T*          alpha,
+                                BatchPtr<const T> A,
+                                int64_t           lda,
+                                BatchPtr<T>       B,
+                                int64_t           ldb,
+                                int64_t           batch_count)
+{
+    constexpr int64_t i32max = 0x7fffffff;
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(m <= 0 || n <= 0 || batch_count <= 0)
+        return rocblas_status_success;
+    const bool sideL = side == rocblas_side_left;
+    if((sideL ? m : n) > i32max)
+        return rocblas_status_invalid_size; // exceeds practical memory
+    if(lda > i32max || ldb > i32max)
+        return rocblas_status_not_implemented;
+    const int64_t nv   = sideL ? n : m;
+    const int64_t step = int64_t(1) << 30;
+    for(int64_t b0 = 0; b0 < batch_count; b0 += step)
+    {
+        const int64_t     bc = std::min(step, batch_count - b0);
+        BatchPtr<const T> Ab = A.arr ? BatchPtr<const T>{A.arr + b0, nullptr, 0, A.offset}
+                                     : BatchPtr<const T>{nullptr, A.base, A.stride,
+                                                         A.offset + b0 * A.stride};
+        BatchPtr<T> Bb = B.arr ? BatchPtr<T>{B.arr + b0, nullptr, 0, B.offset}
+                               : BatchPtr<T>{nullptr, B.base, B.stride, B.offset + b0 * B.stride};
+        for(int64_t v0 = 0; v0 < nv; v0 += step)
+        {
+            const int64_t        vc = std::min(step, nv - v0);
+            const rocblas_status st = internal_trsm<T>(
+                handle, side, uplo, transA, diag, sideL ? int(m) : int(vc),
+                sideL ? int(vc) : int(n), alpha, Ab, int(lda), Bb.shifted(sideL ? v0 * ldb : v0),
+                int(ldb), int(bc));
+            if(st != rocblas_status_success)
+                return st;
+        }
+    }
+    return rocblas_status_success;
 }
 
 rocblas_status no_workspace(size_t* w_x_tmp_size,
@@ -190,6 +247,155 @@ try
     return internal_trsm<T>(handle, side, uplo, transA, diag, m, n, alpha,
                             BatchPtr<const T>{A, nullptr, 0, offset_A}, lda,
                             BatchPtr<T>{B, nullptr, 0, offset_B}, ldb, batch_count);
+}
+catch(...)
+{
+    return rocblas_status_internal_error;
+}
+
+template <typename T>
+rocblas_status rocblas_internal_trsm_workspace_size_64(rocblas_side,
+                                                       rocblas_operation,
+                                                       int64_t,
+                                                       int64_t,
+                                                       int64_t,
+                                                       int64_t,
+                                                       int64_t,
+                                                       int64_t,
+                                                       size_t* w_x_tmp_size,
+                                                       size_t* w_x_tmp_arr_size,
+                                                       size_t* w_invA_size,
+                                                       size_t* w_invA_arr_size,
+                                                       size_t* w_x_tmp_size_backup)
+{
+    return no_workspace(w_x_tmp_size, w_x_tmp_arr_size, w_invA_size, w_invA_arr_size,
+                        w_x_tmp_size_backup);
+}
+
+template <typename T>
+rocblas_status rocblas_internal_trsm_batched_workspace_size_64(rocblas_side,
+                                                               rocblas_operation,
+                                                               int64_t,
+                                                               int64_t,
+                                                               int64_t,
+                                                               int64_t,
+                                                               int64_t,
+                                                               int64_t,
+                                                               size_t* w_x_tmp_size,
+                                                               size_t* w_x_tmp_arr_size,
+                                                               size_t* w_invA_size,
+                                                               size_t* w_invA_arr_size,
+                                                               size_t* w_x_tmp_size_backup)
+{
+    return no_workspace(w_x_tmp_size, w_x_tmp_arr_size, w_invA_size, w_invA_arr_size,
+                        w_x_tmp_size_backup);
+}
+
+// The most any call with m <= M, n <= N needs: nothing, so the reference's
+// own check (every smaller size fits within the maximum) holds trivially.
+template <typename T>
+rocblas_status rocblas_internal_trsm_workspace_max_size_64(rocblas_side,
+                                                           int64_t,
+                                                           int64_t,
+                                                           int64_t,
+                                                           size_t* w_x_tmp_size,
+                                                           size_t* w_invA_size,
+                                                           size_t* w_x_tmp_size_backup)
+{
+    if(!w_x_tmp_size || !w_invA_size || !w_x_tmp_size_backup)
+        return rocblas_status_invalid_pointer;
+    *w_x_tmp_size        = 0;
+    *w_invA_size         = 0;
+    *w_x_tmp_size_backup = 0;
+    return rocblas_status_success;
+}
+
+template <typename T>
+rocblas_status rocblas_internal_trsm_batched_workspace_max_size_64(rocblas_side,
+                                                                   int64_t,
+                                                                   int64_t,
+                                                                   int64_t,
+                                                                   size_t* w_x_tmp_size,
+                                                                   size_t* w_x_tmp_arr_size,
+                                                                   size_t* w_invA_size,
+                                                                   size_t* w_invA_arr_size,
+                                                                   size_t* w_x_tmp_size_backup)
+{
+    const rocblas_status st = no_workspace(w_x_tmp_size, w_x_tmp_arr_size, w_invA_size,
+                                           w_invA_arr_size, w_x_tmp_size_backup);
+    return st == rocblas_status_continue ? rocblas_status_success : st;
+}
+
+template <typename T>
+rocblas_status rocblas_internal_trsm_template_64(rocblas_handle    handle,
+                                                 rocblas_side      side,
+                                                 rocblas_fill      uplo,
+                                                 rocblas_operation transA,
+                                                 rocblas_diagonal  diag,
+                                                 int64_t           m,
+                                                 int64_t           n,
+                                                 const T*          alpha,
+                                                 const T*          A,
+                                                 rocblas_stride    offset_A,
+                                                 int64_t           lda,
+                                                 rocblas_stride    stride_A,
+                                                 T*                B,
+                                                 rocblas_stride    offset_B,
+                                                 int64_t           ldb,
+                                                 rocblas_stride    stride_B,
+                                                 int64_t           batch_count,
+                                                 bool,
+                                                 void*,
+                                                 void*,
+                                                 void*,
+                                                 void*,
+                                                 const T*,
+                                                 int64_t,
+                                                 rocblas_stride,
+                                                 rocblas_stride)
+try
+{
+    return internal_trsm_64<T>(handle, side, uplo, transA, diag, m, n, alpha,
+                               BatchPtr<const T>{nullptr, A, stride_A, offset_A}, lda,
+                               BatchPtr<T>{nullptr, B, stride_B, offset_B}, ldb, batch_count);
+}
+catch(...)
+{
+    return rocblas_status_internal_error;
+}
+
+template <typename T>
+rocblas_status rocblas_internal_trsm_batched_template_64(rocblas_handle    handle,
+                                                         rocblas_side      side,
+                                                         rocblas_fill      uplo,
+                                                         rocblas_operation transA,
+                                                         rocblas_diagonal  diag,
+                                                         int64_t           m,
+                                                         int64_t           n,
+                                                         const T*          alpha,
+                                                         const T* const*   A,
+                                                         rocblas_stride    offset_A,
+                                                         int64_t           lda,
+                                                         rocblas_stride,
+                                                         T* const*      B,
+                                                         rocblas_stride offset_B,
+                                                         int64_t        ldb,
+                                                         rocblas_stride,
+                                                         int64_t batch_count,
+                                                         bool,
+                                                         void*,
+                                                         void*,
+                                                         void*,
+                                                         void*,
+                                                         const T* const*,
+                                                         int64_t,
+                                                         rocblas_stride,
+                                                         rocblas_stride)
+try
+{
+    return internal_trsm_64<T>(handle, side, uplo, transA, diag, m, n, alpha,
+                               BatchPtr<const T>{A, nullptr, 0, offset_A}, lda,
+                               BatchPtr<T>{B, nullptr, 0, offset_B}, ldb, batch_count);
 }
 catch(...)
 {
@@ -331,7 +537,31 @@ catch(...)
     template ROCBLAS_AMD_EXPORT rocblas_status rocblas_internal_trtri_batched_template<T_>(       \
         rocblas_handle, rocblas_fill, rocblas_diagonal, rocblas_int, const T_* const*,            \
         rocblas_stride, rocblas_int, rocblas_stride, rocblas_stride, T_* const*, rocblas_stride,  \
-        rocblas_int, rocblas_stride, rocblas_stride, rocblas_int, rocblas_int, T_* const*);
+        rocblas_int, rocblas_stride, rocblas_stride, rocblas_int, rocblas_int, T_* const*);       \
+    template ROCBLAS_AMD_EXPORT rocblas_status rocblas_internal_trsm_workspace_size_64<T_>(       \
+        rocblas_side, rocblas_operation, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t,    \
+        size_t*, size_t*, size_t*, size_t*, size_t*);                                             \
+    template ROCBLAS_AMD_EXPORT rocblas_status                                                    \
+        rocblas_internal_trsm_batched_workspace_size_64<T_>(rocblas_side, rocblas_operation,      \
+                                                            int64_t, int64_t, int64_t, int64_t,   \
+                                                            int64_t, int64_t, size_t*, size_t*,   \
+                                                            size_t*, size_t*, size_t*);           \
+    template ROCBLAS_AMD_EXPORT rocblas_status rocblas_internal_trsm_workspace_max_size_64<T_>(   \
+        rocblas_side, int64_t, int64_t, int64_t, size_t*, size_t*, size_t*);                      \
+    template ROCBLAS_AMD_EXPORT rocblas_status                                                    \
+        rocblas_internal_trsm_batched_workspace_max_size_64<T_>(                                  \
+            rocblas_side, int64_t, int64_t, int64_t, size_t*, size_t*, size_t*, size_t*,          \
+            size_t*);                                                                             \
+    template ROCBLAS_AMD_EXPORT rocblas_status rocblas_internal_trsm_template_64<T_>(             \
+        rocblas_handle, rocblas_side, rocblas_fill, rocblas_operation, rocblas_diagonal, int64_t, \
+        int64_t, const T_*, const T_*, rocblas_stride, int64_t, rocblas_stride, T_*,              \
+        rocblas_stride, int64_t, rocblas_stride, int64_t, bool, void*, void*, void*, void*,       \
+        const T_*, int64_t, rocblas_stride, rocblas_stride);                                      \
+    template ROCBLAS_AMD_EXPORT rocblas_status rocblas_internal_trsm_batched_template_64<T_>(     \
+        rocblas_handle, rocblas_side, rocblas_fill, rocblas_operation, rocblas_diagonal, int64_t, \
+        int64_t, const T_*, const T_* const*, rocblas_stride, int64_t, rocblas_stride,            \
+        T_* const*, rocblas_stride, int64_t, rocblas_stride, int64_t, bool, void*, void*, void*,  \
+        void*, const T_* const*, int64_t, rocblas_stride, rocblas_stride);
 
 RBAMD_INSTANTIATE_INTERNAL(float)
 RBAMD_INSTANTIATE_INTERNAL(double)

@@ -66,6 +66,23 @@ INTERNAL = [
     "rocblas_internal_trtri_batched_template<{T}>(_rocblas_handle*, rocblas_fill_, "
     "rocblas_diagonal_, int, {T} const* const*, long, int, long, long, {T}* const*, long, int, "
     "long, long, int, int, {T}* const*)",
+    "rocblas_internal_trsm_workspace_size_64<{T}>(rocblas_side_, rocblas_operation_, long, long, "
+    "long, long, long, long, unsigned long*, unsigned long*, unsigned long*, unsigned long*, "
+    "unsigned long*)",
+    "rocblas_internal_trsm_batched_workspace_size_64<{T}>(rocblas_side_, rocblas_operation_, "
+    "long, long, long, long, long, long, unsigned long*, unsigned long*, unsigned long*, "
+    "unsigned long*, unsigned long*)",
+    "rocblas_internal_trsm_workspace_max_size_64<{T}>(rocblas_side_, long, long, long, "
+    "unsigned long*, unsigned long*, unsigned long*)",
+    "rocblas_internal_trsm_batched_workspace_max_size_64<{T}>(rocblas_side_, long, long, long, "
+    "unsigned long*, unsigned long*, unsigned long*, unsigned long*, unsigned long*)",
+    "rocblas_internal_trsm_template_64<{T}>(_rocblas_handle*, rocblas_side_, rocblas_fill_, "
+    "rocblas_operation_, rocblas_diagonal_, long, long, {T} const*, {T} const*, long, long, long, "
+    "{T}*, long, long, long, long, bool, void*, void*, void*, void*, {T} const*, long, long, long)",
+    "rocblas_internal_trsm_batched_template_64<{T}>(_rocblas_handle*, rocblas_side_, "
+    "rocblas_fill_, rocblas_operation_, rocblas_diagonal_, long, long, {T} const*, "
+    "{T} const* const*, long, long, long, {T}* const*, long, long, long, long, bool, void*, void*, "
+    "void*, void*, {T} const* const*, long, long, long)",
 ]
 
 
@@ -80,6 +97,59 @@ def test_internal_cxx_exports():
     want = {"rocblas_status_ " + sig.format(T=t) for sig in INTERNAL for t in ("float", "double")}
     want.add("rocblas_internal_trtri_temp_elements(int, int)")
     assert demangled == want, sorted(demangled ^ want)
+
+
+def _internal(prefix, restype, argtypes):
+    out = subprocess.run(["nm", "-D", "--defined-only", rb.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    dem = subprocess.run(["nm", "-DC", "--defined-only", rb.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    for raw, d in zip(out.splitlines(), dem.splitlines()):
+        name = d.split(" ", 2)[2]
+        if name.startswith(prefix) or name.startswith("rocblas_status_ " + prefix):
+            fn = getattr(rb.lib(), raw.split()[-1])
+            fn.restype, fn.argtypes = restype, argtypes
+            return fn
+    raise KeyError(prefix)
+
+
+@pytest.mark.parametrize("T", ["float", "double"])
+def test_internal_workspace_max_size_covers_every_smaller_size(T):
+    """The reference's testing_trsm_internal_interfaces
+    (clients/include/blas3/testing_trsm.hpp:188-240, _batched.hpp:290-350):
+    for every m <= M, n <= N the per-call workspace query (transpose, no
+    supplied invA) asks for no more than the max-size query reported."""
+    sz = ctypes.POINTER(ctypes.c_size_t)
+    i32, i64 = ctypes.c_int32, ctypes.c_int64
+    mx = _internal(f"rocblas_internal_trsm_workspace_max_size_64<{T}>", ctypes.c_int32,
+                   [i32, i64, i64, i64, sz, sz, sz])
+    mxb = _internal(f"rocblas_internal_trsm_batched_workspace_max_size_64<{T}>", ctypes.c_int32,
+                    [i32, i64, i64, i64] + [sz] * 5)
+    ws = _internal(f"rocblas_internal_trsm_workspace_size<{T}>", ctypes.c_int32,
+                   [i32, i32, i32, i32, i32, i32] + [sz] * 5)
+    wsb = _internal(f"rocblas_internal_trsm_batched_workspace_size<{T}>", ctypes.c_int32,
+                    [i32, i32, i32, i32, i32, i32] + [sz] * 5)
+    ws64 = _internal(f"rocblas_internal_trsm_workspace_size_64<{T}>", ctypes.c_int32,
+                     [i32, i32, i64, i64, i64, i64, i64, i64] + [sz] * 5)
+    S = rb.rocblas_status
+    for side in (141, 142):
+        for (M, N, batch) in [(130, 70, 3), (20, 33, 1)]:
+            top = [ctypes.c_size_t(99) for _ in range(5)]
+            assert mx(side, M, N, batch, *[ctypes.byref(top[i]) for i in (0, 2, 4)]) == S.success
+            assert mxb(side, M, N, batch, *[ctypes.byref(x) for x in top]) == S.success
+            assert all(x.value == 0 for x in top)
+            for m in range(M, 0, -7):
+                for n in range(N, 0, -5):
+                    got = [ctypes.c_size_t(7) for _ in range(5)]
+                    refs = [ctypes.byref(x) for x in got]
+                    for st in (ws(side, 112, m, n, batch, 0, *refs),
+                               wsb(side, 112, m, n, batch, 0, *refs),
+                               ws64(side, 112, m, n, max(m, n), m, batch, 0, *refs)):
+                        assert st in (S.success, S.continue_)
+                        assert got[0].value <= top[0].value and got[2].value <= top[2].value
+                        assert got[4].value <= top[4].value
+    got = [ctypes.c_size_t(0) for _ in range(3)]
+    assert mx(141, 8, 8, 1, None, ctypes.byref(got[1]), ctypes.byref(got[2])) == S.invalid_pointer
 
 
 def test_internal_trtri_temp_elements():

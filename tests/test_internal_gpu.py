@@ -195,3 +195,53 @@ def test_trtri_template_sub_batches_and_in_place(h):
             inv = one.cpu().numpy().reshape(n, lda).T[:n]
             tri = np.tril if uplo == "L" else np.triu
             assert np.array_equal(tri(got), tri(inv)), o
+
+
+TRSM64_ARGS = [_vp, _i32, _i32, _i32, _i32, _i64, _i64, _vp, _vp, _i64, _i64, _i64, _vp, _i64,
+               _i64, _i64, _i64, ctypes.c_bool, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64]
+
+
+def test_trsm_template_64_matches_32_bit_form(h):
+    """The ILP64 templates (library/src/src64/blas3/rocblas_trsm_kernels_64.cpp:31-225)
+    give the 32-bit templates' results bit for bit on sizes both accept; a
+    triangle beyond INT32_MAX is invalid_size and a leading dimension beyond it
+    not_implemented, both before any launch."""
+    rng = np.random.default_rng(14)
+    f32 = fn("rocblas_status_ rocblas_internal_trsm_template<double>", TRSM_ARGS)
+    f64 = fn("rocblas_status_ rocblas_internal_trsm_template_64<double>", TRSM64_ARGS)
+    fb64 = fn("rocblas_status_ rocblas_internal_trsm_batched_template_64<double>", TRSM64_ARGS)
+    alpha = ctypes.c_double(-1.5)
+    for side, uplo, trans, m, n, batch in [("L", "U", "N", 300, 20, 2), ("R", "L", "T", 33, 90, 3)]:
+        say("internal trsm_64", side, uplo, trans, m, n, batch)
+        k = m if side == "L" else n
+        lda, ldb = k, m
+        sA, sB = lda * k, ldb * n
+        A = np.concatenate([oracle.diag_dominant_triangular(k, uplo, rng).ravel(order="F")
+                            for _ in range(batch)])
+        B = rng.uniform(-1, 1, sB * batch)
+        dA = torch.from_numpy(A).cuda()
+        want = torch.from_numpy(B.copy()).cuda()
+        got = torch.from_numpy(B.copy()).cuda()
+        got2 = torch.from_numpy(B.copy()).cuda()
+        args = (SIDE[side], FILL[uplo], OPS[trans], DIAG["N"], m, n, ctypes.addressof(alpha))
+        assert f32(h.ptr, *args, dA.data_ptr(), 0, lda, sA, want.data_ptr(), 0, ldb, sB, batch,
+                   True, None, None, None, None, None, 0, 0, 0) == S.success
+        assert f64(h.ptr, *args, dA.data_ptr(), 0, lda, sA, got.data_ptr(), 0, ldb, sB, batch,
+                   True, None, None, None, None, None, 0, 0, 0) == S.success
+        pa = ptr_array([dA[b * sA:] for b in range(batch)])
+        pb = ptr_array([got2[b * sB:] for b in range(batch)])
+        assert fb64(h.ptr, *args, pa.data_ptr(), 0, lda, 0, pb.data_ptr(), 0, ldb, 0, batch,
+                    True, None, None, None, None, None, 0, 0, 0) == S.success
+        torch.cuda.synchronize()
+        assert torch.equal(got, want) and torch.equal(got2, want)
+    big = 1 << 31
+    one = ctypes.c_double(1.0)
+    common = (ctypes.addressof(one), dA.data_ptr(), 0)
+    assert f64(h.ptr, 141, 122, 111, 131, big, 4, *common, big, 0, got.data_ptr(), 0, big, 0, 1,
+               True, None, None, None, None, None, 0, 0, 0) == S.invalid_size
+    assert f64(h.ptr, 142, 122, 111, 131, 4, big, *common, big, 0, got.data_ptr(), 0, 4, 0, 1,
+               True, None, None, None, None, None, 0, 0, 0) == S.invalid_size
+    assert f64(h.ptr, 141, 122, 111, 131, 8, 4, *common, big, 0, got.data_ptr(), 0, 8, 0, 1,
+               True, None, None, None, None, None, 0, 0, 0) == S.not_implemented
+    assert f64(h.ptr, 141, 122, 111, 131, 0, 4, *common, 8, 0, got.data_ptr(), 0, 8, 0, 1,
+               True, None, None, None, None, None, 0, 0, 0) == S.success
