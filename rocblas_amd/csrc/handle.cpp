@@ -3,6 +3,7 @@
 // library/src/rocblas_auxiliary.cpp and library/src/handle.cpp (cited per
 // function); the implementation is this library's own.
 #include "internal.hpp"
+#include "logging.hpp"
 
 #include <cstdlib>
 #include <new>
@@ -149,6 +150,7 @@ try
         delete h;
         return rocblas_status_memory_error;
     }
+    rbamd::init_logging(h);
     *handle = h;
     return rocblas_status_success;
 }

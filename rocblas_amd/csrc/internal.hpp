@@ -21,6 +21,11 @@ enum class WorkspaceOwner
     user_owned
 };
 
+namespace rbamd
+{
+struct LogSink;
+}
+
 struct _rocblas_handle
 {
     int                  device        = 0;
@@ -38,6 +43,11 @@ struct _rocblas_handle
     // completion counters of the chained solve (chain_trsm.hip): one per
     // panel, then a sticky error word; zero between launches
     void*                chain_flags   = nullptr;
+    // ROCBLAS_LAYER bits read at creation and the streams they opened
+    int                  layer_mode     = 0;
+    rbamd::LogSink*      log_trace_os   = nullptr;
+    rbamd::LogSink*      log_bench_os   = nullptr;
+    rbamd::LogSink*      log_profile_os = nullptr;
 };
 
 namespace rbamd

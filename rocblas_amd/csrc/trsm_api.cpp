@@ -11,7 +11,12 @@
 // data type at run time and an optional invA of packed diagonal-block
 // inverses.  The solve here forms no inverses, so a supplied invA is accepted
 // and not read; the result is the same solve as the typed entry points.
+//
+// Every entry point logs its call per ROCBLAS_LAYER (logging.hpp) before the
+// argument check, as the reference does; the _ex forms log as the typed
+// routine they compute (the reference routes them through the same _impl).
 #include "internal.hpp"
+#include "logging.hpp"
 
 #include <algorithm>
 #include <new>
@@ -216,6 +221,8 @@ using namespace rbamd;
                         rocblas_int       ldb)                                                  \
     {                                                                                           \
         RBAMD_TRY_BEGIN                                                                         \
+        log_trsm<T>(handle, TrsmForm::plain, false, side, uplo, transA, diag, m, n, alpha, A, lda, \
+                    0, B, ldb, 0, 1);                                                           \
         return trsm_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                     \
                              BatchPtr<const T>{nullptr, A, 0, 0}, A == nullptr, lda,            \
                              BatchPtr<T>{nullptr, B, 0, 0}, B == nullptr, ldb, 1);              \
@@ -238,6 +245,8 @@ using namespace rbamd;
                         rocblas_int       batch_count)                                          \
     {                                                                                           \
         RBAMD_TRY_BEGIN                                                                         \
+        log_trsm<T>(handle, TrsmForm::batched, false, side, uplo, transA, diag, m, n, alpha, A,  \
+                    lda, 0, B, ldb, 0, batch_count);                                            \
         return trsm_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                     \
                              BatchPtr<const T>{A, nullptr, 0, 0}, A == nullptr, lda,            \
                              BatchPtr<T>{B, nullptr, 0, 0}, B == nullptr, ldb, batch_count);    \
@@ -262,6 +271,8 @@ using namespace rbamd;
                         rocblas_int       batch_count)                                          \
     {                                                                                           \
         RBAMD_TRY_BEGIN                                                                         \
+        log_trsm<T>(handle, TrsmForm::strided, false, side, uplo, transA, diag, m, n, alpha, A,  \
+                    lda, stride_a, B, ldb, stride_b, batch_count);                              \
         return trsm_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                     \
                              BatchPtr<const T>{nullptr, A, stride_a, 0}, A == nullptr, lda,     \
                              BatchPtr<T>{nullptr, B, stride_b, 0}, B == nullptr, ldb,           \
@@ -307,6 +318,8 @@ rocblas_status rocblas_trsm_ex(rocblas_handle    handle,
     (void)invA_size;
     RBAMD_TRY_BEGIN
     RBAMD_TRSM_EX_DISPATCH(
+        log_trsm<T>(handle, TrsmForm::plain, false, side, uplo, transA, diag, m, n,
+                    static_cast<const T*>(alpha), A, lda, 0, B, ldb, 0, 1);
         return trsm_entry<T>(handle, side, uplo, transA, diag, m, n,
                              static_cast<const T*>(alpha),
                              BatchPtr<const T>{nullptr, static_cast<const T*>(A), 0, 0},
@@ -336,6 +349,8 @@ rocblas_status rocblas_trsm_batched_ex(rocblas_handle    handle,
     (void)invA_size;
     RBAMD_TRY_BEGIN
     RBAMD_TRSM_EX_DISPATCH(
+        log_trsm<T>(handle, TrsmForm::batched, false, side, uplo, transA, diag, m, n,
+                    static_cast<const T*>(alpha), A, lda, 0, B, ldb, 0, batch_count);
         return trsm_entry<T>(handle, side, uplo, transA, diag, m, n,
                              static_cast<const T*>(alpha),
                              BatchPtr<const T>{static_cast<const T* const*>(A), nullptr, 0, 0},
@@ -370,6 +385,9 @@ rocblas_status rocblas_trsm_strided_batched_ex(rocblas_handle    handle,
     (void)stride_invA;
     RBAMD_TRY_BEGIN
     RBAMD_TRSM_EX_DISPATCH(
+        log_trsm<T>(handle, TrsmForm::strided, false, side, uplo, transA, diag, m, n,
+                    static_cast<const T*>(alpha), A, lda, stride_A, B, ldb, stride_B,
+                    batch_count);
         return trsm_entry<T>(handle, side, uplo, transA, diag, m, n,
                              static_cast<const T*>(alpha),
                              BatchPtr<const T>{nullptr, static_cast<const T*>(A), stride_A, 0},
@@ -394,6 +412,8 @@ rocblas_status rocblas_trsm_strided_batched_ex(rocblas_handle    handle,
                         int64_t           ldb)                                                   \
     {                                                                                            \
         RBAMD_TRY_BEGIN                                                                          \
+        log_trsm<T>(handle, TrsmForm::plain, true, side, uplo, transA, diag, m, n, alpha, A, lda, \
+                    0, B, ldb, 0, 1);                                                            \
         return trsm64_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                    \
                                BatchPtr<const T>{nullptr, A, 0, 0}, A == nullptr, lda,           \
                                BatchPtr<T>{nullptr, B, 0, 0}, B == nullptr, ldb, 1);             \
@@ -416,6 +436,8 @@ rocblas_status rocblas_trsm_strided_batched_ex(rocblas_handle    handle,
                         int64_t           batch_count)                                           \
     {                                                                                            \
         RBAMD_TRY_BEGIN                                                                          \
+        log_trsm<T>(handle, TrsmForm::batched, true, side, uplo, transA, diag, m, n, alpha, A,    \
+                    lda, 0, B, ldb, 0, batch_count);                                             \
         return trsm64_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                    \
                                BatchPtr<const T>{A, nullptr, 0, 0}, A == nullptr, lda,           \
                                BatchPtr<T>{B, nullptr, 0, 0}, B == nullptr, ldb, batch_count);   \
@@ -440,6 +462,8 @@ rocblas_status rocblas_trsm_strided_batched_ex(rocblas_handle    handle,
                         int64_t           batch_count)                                           \
     {                                                                                            \
         RBAMD_TRY_BEGIN                                                                          \
+        log_trsm<T>(handle, TrsmForm::strided, true, side, uplo, transA, diag, m, n, alpha, A,    \
+                    lda, stride_a, B, ldb, stride_b, batch_count);                               \
         return trsm64_entry<T>(handle, side, uplo, transA, diag, m, n, alpha,                    \
                                BatchPtr<const T>{nullptr, A, stride_a, 0}, A == nullptr, lda,    \
                                BatchPtr<T>{nullptr, B, stride_b, 0}, B == nullptr, ldb,          \
