@@ -252,6 +252,8 @@ def case_kind(case):
         return f"precision {prec} not built (real single and double only)"
     if case.get("HMM"):
         return "HipManagedMemory cases not run"
+    if max(int(case.get("lda", 0)), int(case.get("ldb", 0))) > 0x7FFFFFFF:
+        return "leading dimension beyond INT32_MAX: not implemented by the _64 entry points"
     if case.get("threads", 0) or case.get("streams", 0) or case.get("devices", 0):
         return "multi-thread / multi-stream / multi-device cases not run"
     return "run"
@@ -260,41 +262,59 @@ def case_kind(case):
 # --------------------------------------------------------------------------
 # data (the reference's generators, restated)
 # --------------------------------------------------------------------------
-def _rand(rng, init, shape, dtype):
+def _rand(rng, init, shape, dtype, rows=None, cols=None, ld=0):
+    """Values of the reference's initialisations: rand_int 1..10, hpl
+    U(-0.5, 0.5), trig_float sin of the element's linear index (rows / cols
+    give the block's absolute indices for that)."""
     if init == "hpl":
         return rng.uniform(-0.5, 0.5, shape).astype(dtype)
     if init == "trig_float":
-        i = np.arange(int(np.prod(shape))).reshape(shape)
-        return np.sin(i).astype(dtype)
-    return rng.integers(1, 11, shape).astype(dtype)  # rand_int: 1..10
+        r = np.arange(shape[0]) if rows is None else rows
+        c = np.arange(shape[1]) if cols is None else cols
+        return np.sin(r[:, None] + c[None, :] * max(ld, shape[0])).astype(dtype)
+    return rng.integers(1, 11, shape, dtype=np.int8).astype(dtype)  # rand_int: 1..10
 
 
 def init_triangle(arg, rng):
-    """lda x K column-major diagonally dominant triangle (rocblas_init.hpp:263-327)."""
+    """lda x K column-major diagonally dominant triangle (rocblas_init.hpp:263-327),
+    generated in blocks of columns so large K needs little more than A itself."""
     K, lda, dt = arg.K, arg.lda, arg.dtype
+    upper = arg.uplo.upper() == "U"
     A = np.zeros((lda, K), dtype=dt, order="F")
-    v = _rand(rng, arg.initialization, (K, K), dt)
-    tri = np.triu(v) if arg.uplo.upper() == "U" else np.tril(v)
-    off = np.abs(tri - np.diag(np.diag(tri)))
+    rowsum = np.zeros(K)
+    colsum = np.zeros(K)
+    step = max(1, (1 << 23) // max(K, 1))
+    rows = np.arange(K)
+    for j0 in range(0, K, step):
+        j1 = min(K, j0 + step)
+        cols = np.arange(j0, j1)
+        keep = (cols[None, :] >= rows[:, None]) if upper else (cols[None, :] <= rows[:, None])
+        v = np.where(keep, _rand(rng, arg.initialization, (K, j1 - j0), dt, rows, cols, lda), 0)
+        A[:K, j0:j1] = v
+        off = np.abs(np.where(cols[None, :] == rows[:, None], 0, v), dtype=np.float64)
+        colsum[j0:j1] += off.sum(axis=0)
+        rowsum += off.sum(axis=1)
     # row sum + column sum of the off-diagonal entries, times 1.01
-    s = off.sum(axis=0) + off.sum(axis=1)
-    d = np.where(s == 0, dt(1), (s * dt(1.01)).astype(dt))
-    np.fill_diagonal(tri, d)
+    s = rowsum + colsum
+    d = np.where(s == 0, 1.0, s * 1.01).astype(dt)
+    A[rows, rows] = d
     if arg.diag.upper() == "U":
-        # client_utility.hpp:419-444: scale by the diagonal, then overwrite it
-        if arg.uplo.upper() == "L":
-            tri = tri / np.diag(tri)[:, None]
-        else:
-            tri = tri / np.diag(tri)[None, :]
-        np.fill_diagonal(tri, _rand(rng, arg.initialization, (K,), dt))
-    A[:K, :K] = tri.astype(dt)
+        # client_utility.hpp:419-444: scale rows (lower) or columns (upper)
+        # by the diagonal, then overwrite the diagonal with random values
+        for j0 in range(0, K, step):
+            j1 = min(K, j0 + step)
+            if upper:
+                A[:K, j0:j1] /= d[None, j0:j1]
+            else:
+                A[:K, j0:j1] /= d[:, None]
+        A[rows, rows] = _rand(rng, arg.initialization, (1, K), dt)[0]
     return A
 
 
 def init_general(arg, rng, m, n, ld):
     """ld x n column-major matrix, alternating signs (rocblas_init.hpp:63-84)."""
     X = np.zeros((ld, n), dtype=arg.dtype, order="F")
-    v = _rand(rng, arg.initialization, (m, n), arg.dtype)
+    v = _rand(rng, arg.initialization, (m, n), arg.dtype, ld=ld)
     i, j = np.indices((m, n))
     X[:m, :n] = np.where((i ^ j) & 1, v, -v)
     return X

@@ -10,6 +10,7 @@
 // each XCD's L2 sees contiguous runs of tiles, grouped by row panel.
 #include "device_common.hpp"
 
+#include <cstdlib>
 #include <type_traits>
 
 namespace rbamd
@@ -228,6 +229,215 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 GEMM with the operand tiles loaded straight into LDS
+// (global_load_lds_dwordx4): C := alpha * A * B + beta * C, A and B not
+// transposed, dense, 128 x 128 tiles, K steps of 8, three LDS stages with two
+// tiles in flight.  No staging registers, so the deeper pipeline costs no
+// occupancy.  Each wavefront's load instruction writes 64 x 16 contiguous
+// LDS bytes, lane l at byte 16 l:
+//   A stage: 8 rows (k) of 128 doubles (i), row pitch 144 doubles, so the
+//            fragment reads of two k rows fall in disjoint bank halves;
+//            one instruction is one row, lane l holding rows i = 2l, 2l + 1.
+//   B stage: 128 rows (j) of 8 doubles (k); 16 rows per instruction, lane l
+//            holding row l / 4 and the k pair (l % 4) ^ ((row / 4) % 4), an
+//            XOR swizzle that makes the 16 rows a fragment read touches fall
+//            in distinct banks.
+// Eligible when M, lda, ldb and every operand address are even / 16-byte
+// aligned and K is a multiple of 8 (launch_gemm checks); rows of A beyond M
+// and columns of B beyond N are read clamped and only feed rows / columns of
+// C that are not stored.
+// ---------------------------------------------------------------------------
+constexpr int DMA_LDA   = 144;
+constexpr int DMA_A_ELS = 8 * DMA_LDA;
+constexpr int DMA_B_ELS = 128 * 8;
+
+// One global_load_lds_dwordx4, written as inline assembly so the compiler's
+// wait-count pass does not treat every later LDS read as depending on it (it
+// cannot tell the stages apart and would wait for all loads in flight); the
+// kernel's explicit vmcnt waits order it instead.  M0 holds the wavefront's
+// LDS destination; one wait state separates its write from the load.
+__device__ __forceinline__ void dma16(const double* src, const double* lds)
+{
+    // wave-uniform (derived from the wavefront index); readfirstlane puts it
+    // in the scalar register the "s" constraint needs
+    const uint32_t dst = __builtin_amdgcn_readfirstlane(uint32_t(
+        reinterpret_cast<uintptr_t>((const __attribute__((address_space(3))) double*)(lds))));
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+                 :
+                 : "v"(src), "s"(dst)
+                 : "memory", "m0");
+}
+
+// wait until at most N of this wavefront's vector-memory loads are in flight
+// (gfx9 s_waitcnt encoding; expcnt and lgkmcnt left at their maxima)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 16, "vmcnt");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | N);
+}
+
+// Workgroup barrier without __syncthreads' memory fence, whose vmcnt(0)
+// would also wait for the loads of the tiles still in flight; the "memory"
+// clobber keeps LDS accesses on their side of it.  The LDS reads before it
+// are complete (their values feed MFMAs already issued) and wait_vmcnt has
+// retired this wavefront's loads of the tile the next reads use.
+__device__ __forceinline__ void barrier_no_fence()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+__global__ void __launch_bounds__(256) gemm_dma_kernel(const GemmArgsT<double, double, double> a)
+{
+    __shared__ __align__(16) double As0[DMA_A_ELS];
+    __shared__ __align__(16) double As1[DMA_A_ELS];
+    __shared__ __align__(16) double As2[DMA_A_ELS];
+    __shared__ __align__(16) double Bs0[DMA_B_ELS];
+    __shared__ __align__(16) double Bs1[DMA_B_ELS];
+    __shared__ __align__(16) double Bs2[DMA_B_ELS];
+
+    if(a.skip_if_zero && *a.skip_if_zero == 0.0)
+        return;
+
+    const int tid  = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int w    = tid / WAVE;
+    const int wm   = w % 2;
+    const int wn   = w / 2;
+
+    // tile index: the same XCD-contiguous, row-panel-grouped order as gemm_kernel
+    const int total = a.tiles_m * a.tiles_n;
+    int       bid   = blockIdx.x;
+    {
+        const int q = total / 8, r = total % 8;
+        const int x = bid % 8, l = bid / 8;
+        bid         = x * q + min(x, r) + l;
+    }
+    constexpr int GROUP = 8;
+    const int     grp   = bid / (GROUP * a.tiles_n);
+    const int     fm    = grp * GROUP;
+    const int     gsz   = min(a.tiles_m - fm, GROUP);
+    const int     m0    = (fm + (bid % gsz)) * 128;
+    const int     n0    = ((bid % (GROUP * a.tiles_n)) / gsz) * 128;
+    const int     ktiles = a.K / 8;
+
+    // this lane's load sources, fixed over K (plus k0 * lda for A, k0 for B)
+    const int ai = min(m0 + 2 * lane, a.M - 2);
+    const int jl = lane >> 2;
+    const int kp = (lane & 3) ^ ((jl >> 2) & 3);
+    const int j0 = min(n0 + (2 * w) * 16 + jl, a.N - 1);
+    const int j1 = min(n0 + (2 * w + 1) * 16 + jl, a.N - 1);
+
+    for(int b = blockIdx.y; b < a.batch; b += gridDim.y)
+    {
+        const double* Ap  = a.A.get(b);
+        const double* Bp  = a.B.get(b);
+        double*       Cp  = a.C.get(b);
+        const double* sA0 = Ap + ai + int64_t(2 * w) * a.lda;
+        const double* sA1 = Ap + ai + int64_t(2 * w + 1) * a.lda;
+        const double* sB0 = Bp + 2 * kp + int64_t(j0) * a.ldb;
+        const double* sB1 = Bp + 2 * kp + int64_t(j1) * a.ldb;
+
+        // tiles past the last one re-read the last tile (in bounds) into the
+        // stage nobody reads any more, so every step waits the same way
+        auto issue = [&](int kt, double* As, double* Bs) {
+            const int64_t k0 = int64_t(min(kt, ktiles - 1)) * 8;
+            dma16(sA0 + k0 * a.lda, As + (2 * w) * DMA_LDA);
+            dma16(sA1 + k0 * a.lda, As + (2 * w + 1) * DMA_LDA);
+            dma16(sB0 + k0, Bs + (2 * w) * 128);
+            dma16(sB1 + k0, Bs + (2 * w + 1) * 128);
+        };
+
+        d4_t acc[4][4];
+#pragma unroll
+        for(int i = 0; i < 4; ++i)
+#pragma unroll
+            for(int j = 0; j < 4; ++j)
+                acc[i][j] = d4_t{0, 0, 0, 0};
+
+        auto compute = [&](const double* as, const double* bs) {
+#pragma unroll
+            for(int k4 = 0; k4 < 2; ++k4)
+            {
+                const int kr = k4 * 4 + lane / 16;
+                double    af[4], bf[4];
+#pragma unroll
+                for(int i = 0; i < 4; ++i)
+                    af[i] = as[kr * DMA_LDA + wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+                for(int j = 0; j < 4; ++j)
+                {
+                    const int jj = wn * 64 + j * 16 + (lane & 15);
+                    bf[j] = bs[jj * 8 + ((((kr >> 1) ^ ((jj >> 2) & 3))) << 1) + (kr & 1)];
+                }
+#pragma unroll
+                for(int i = 0; i < 4; ++i)
+#pragma unroll
+                    for(int j = 0; j < 4; ++j)
+                        acc[i][j] = Mfma<double>::run(bf[j], af[i], acc[i][j]);
+            }
+        };
+        // one K step: tiles kt and kt + 1 are in flight (four loads each), so
+        // vmcnt(4) retires tile kt; the barrier publishes it to every
+        // wavefront and retires every read of the stage tile kt + 2 reuses
+        auto step = [&](int kt, const double* as, const double* bs, double* an, double* bn) {
+            wait_vmcnt<4>();
+            barrier_no_fence();
+            issue(kt + 2, an, bn);
+            compute(as, bs);
+        };
+
+        if(ktiles > 0)
+        {
+            issue(0, As0, Bs0);
+            issue(1, As1, Bs1);
+            for(int kt = 0; kt < ktiles; kt += 3)
+            {
+                step(kt, As0, Bs0, As2, Bs2);
+                if(kt + 1 < ktiles)
+                    step(kt + 1, As1, Bs1, As0, Bs0);
+                if(kt + 2 < ktiles)
+                    step(kt + 2, As2, Bs2, As1, Bs1);
+            }
+            wait_vmcnt<0>(); // the two tiles past the end
+        }
+
+        // epilogue: as gemm_kernel's
+#pragma unroll
+        for(int i = 0; i < 4; ++i)
+        {
+            const int row = m0 + wm * 64 + i * 16 + (lane & 15);
+#pragma unroll
+            for(int j = 0; j < 4; ++j)
+#pragma unroll
+                for(int r = 0; r < 4; ++r)
+                {
+                    const int col = n0 + wn * 64 + j * 16 + Mfma<double>::row(lane, r);
+                    if(row < a.M && col < a.N)
+                    {
+                        double& cref = Cp[row + int64_t(col) * a.ldc];
+                        double  v    = a.alpha * acc[i][j][r];
+                        if(a.beta != 0.0)
+                            v = fma(a.beta, cref, v);
+                        cref = v;
+                    }
+                }
+        }
+        __syncthreads(); // LDS reuse across batch iterations
+    }
+}
+
+// The DMA kernel's conditions: plain FP64 operands, 16-byte aligned rows.
+static bool gemm_dma_eligible(const GemmArgsT<double, double, double>& a)
+{
+    auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    return a.opA == 0 && a.opB == 0 && !a.tri && a.M >= 2 && a.M % 2 == 0 && a.K >= 8
+           && a.K % 8 == 0 && a.lda % 2 == 0 && a.ldb % 2 == 0 && !a.A.arr && !a.B.arr
+           && al16(a.A.base + a.A.offset) && al16(a.B.base + a.B.offset) && a.A.stride % 2 == 0
+           && a.B.stride % 2 == 0;
+}
+
 // MFMA type of an operand pair: FP64 only when both operands are FP64.
 template <typename TA, typename TB>
 using mfma_t = std::conditional_t<std::is_same_v<TA, double> && std::is_same_v<TB, double>,
@@ -299,6 +509,16 @@ static rocblas_status launch_gemm_cfg(rocblas_handle h, GemmArgsT<TA, TB, TC> a)
     return launch_status();
 }
 
+// ROCBLAS_AMD_GEMM_DMA=0 keeps every GEMM on gemm_kernel (A/B comparisons).
+static bool gemm_dma_disabled()
+{
+    static const bool off = [] {
+        const char* e = std::getenv("ROCBLAS_AMD_GEMM_DMA");
+        return e && e[0] == '0';
+    }();
+    return off;
+}
+
 // Pick the largest tile that still gives every CU work.
 template <typename TA, typename TB, typename TC>
 rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
@@ -319,6 +539,19 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
     {
         // FP64 runs best with K steps of 8 (two waves per SIMD), FP32 with 16
         // (measured at 8192x16384x8192 on MI355X: 65 vs 50 TF/s, 96 vs 94 TF/s)
+        if constexpr(std::is_same_v<TA, double> && std::is_same_v<TB, double>
+                     && std::is_same_v<TC, double>)
+        {
+            if(gemm_dma_eligible(a) && !gemm_dma_disabled())
+            {
+                GemmArgsT<TA, TB, TC> d = a;
+                d.tiles_m               = (a.M + 127) / 128;
+                d.tiles_n               = (a.N + 127) / 128;
+                const dim3 grid(unsigned(int64_t(d.tiles_m) * d.tiles_n), grid_batch(a.batch));
+                hipLaunchKernelGGL(gemm_dma_kernel, grid, dim3(256), 0, h->stream, d);
+                return launch_status();
+            }
+        }
         if constexpr(std::is_same_v<mfma_t<TA, TB>, double>)
             return launch_gemm_cfg<TA, TB, TC, 128, 128, 8, 2, 2>(h, a);
         else

@@ -50,7 +50,8 @@ def run(args, handle, verbose=False, out=None, progress=0):
             print(f"[ PROGRESS ] {i}/{len(args)}, {len(failures)} failed", file=out, flush=True)
         arg.seed = i
         label = arg.label()
-        if verbose:
+        # a large case can take a while on the host: name it before it starts
+        if verbose or C.case_bytes(arg) > (1 << 30) or arg.K > 8192:
             print(f"[ RUN      ] {label}", file=out, flush=True)
         try:
             C.run_case(handle, arg)
