@@ -232,31 +232,31 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
 // ---------------------------------------------------------------------------
 // FP64 GEMM with the operand tiles loaded straight into LDS
 // (global_load_lds_dwordx4): C := alpha * A * B + beta * C, A and B not
-// transposed, dense, 128 x 128 tiles, K steps of 8, three LDS stages with two
-// tiles in flight.  No staging registers, so the deeper pipeline costs no
-// occupancy.  Each wavefront's load instruction writes 64 x 16 contiguous
-// LDS bytes, lane l at byte 16 l:
-//   A stage: 8 rows (k) of 128 doubles (i), row pitch 144 doubles, so the
+// transposed, dense, 128 x 128 tiles, K steps of BK (8 or 16) and NS LDS
+// stages with NS - 1 tiles in flight.  No staging registers, so the deeper
+// pipeline costs no occupancy.  Each wavefront's load instruction writes
+// 64 x 16 contiguous LDS bytes, lane l at byte 16 l:
+//   A stage: BK rows (k) of 128 doubles (i), row pitch 144 doubles, so the
 //            fragment reads of two k rows fall in disjoint bank halves;
 //            one instruction is one row, lane l holding rows i = 2l, 2l + 1.
-//   B stage: 128 rows (j) of 8 doubles (k); 16 rows per instruction, lane l
-//            holding row l / 4 and the k pair (l % 4) ^ ((row / 4) % 4), an
-//            XOR swizzle that makes the 16 rows a fragment read touches fall
-//            in distinct banks.
+//   B stage: 128 rows (j) of BK doubles (k), 128 / BK rows per instruction;
+//            lane l holds row l / (BK/2) and the k pair
+//            (l % (BK/2)) ^ key(j), an XOR swizzle by j / (256 / (8 BK))
+//            that puts the 16 rows a fragment read touches in distinct banks.
 // Eligible when M, lda, ldb and every operand address are even / 16-byte
-// aligned and K is a multiple of 8 (launch_gemm checks); rows of A beyond M
+// aligned and K is a multiple of BK (launch_gemm checks); rows of A beyond M
 // and columns of B beyond N are read clamped and only feed rows / columns of
 // C that are not stored.
 // ---------------------------------------------------------------------------
-constexpr int DMA_LDA   = 144;
-constexpr int DMA_A_ELS = 8 * DMA_LDA;
-constexpr int DMA_B_ELS = 128 * 8;
+constexpr int DMA_LDA = 144;
 
 // One global_load_lds_dwordx4, written as inline assembly so the compiler's
 // wait-count pass does not treat every later LDS read as depending on it (it
 // cannot tell the stages apart and would wait for all loads in flight); the
 // kernel's explicit vmcnt waits order it instead.  M0 holds the wavefront's
 // LDS destination; one wait state separates its write from the load.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm" // m0 is reserved; the clobber is still honoured
 __device__ __forceinline__ void dma16(const double* src, const double* lds)
 {
     // wave-uniform (derived from the wavefront index); readfirstlane puts it
@@ -268,6 +268,7 @@ __device__ __forceinline__ void dma16(const double* src, const double* lds)
                  : "v"(src), "s"(dst)
                  : "memory", "m0");
 }
+#pragma clang diagnostic pop
 
 // wait until at most N of this wavefront's vector-memory loads are in flight
 // (gfx9 s_waitcnt encoding; expcnt and lgkmcnt left at their maxima)
@@ -288,14 +289,19 @@ __device__ __forceinline__ void barrier_no_fence()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-__global__ void __launch_bounds__(256) gemm_dma_kernel(const GemmArgsT<double, double, double> a)
+template <int BK, int NS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    gemm_dma_kernel(const GemmArgsT<double, double, double> a)
 {
-    __shared__ __align__(16) double As0[DMA_A_ELS];
-    __shared__ __align__(16) double As1[DMA_A_ELS];
-    __shared__ __align__(16) double As2[DMA_A_ELS];
-    __shared__ __align__(16) double Bs0[DMA_B_ELS];
-    __shared__ __align__(16) double Bs1[DMA_B_ELS];
-    __shared__ __align__(16) double Bs2[DMA_B_ELS];
+    static_assert((BK == 8 || BK == 16) && NS >= 2 && NS <= 3, "configuration");
+    constexpr int A_ELS = BK * DMA_LDA;
+    constexpr int B_ELS = 128 * BK;
+    constexpr int NSLOT = BK / 2;           // k pairs per B row
+    constexpr int JPC   = 256 / (8 * BK);   // B rows per 256-byte bank cycle
+    constexpr int PER   = BK / 4;           // A rows / B groups per wavefront
+    constexpr int LOADS = 2 * PER;          // loads per wavefront per tile
+    __shared__ __align__(16) double As[NS][A_ELS];
+    __shared__ __align__(16) double Bs[NS][B_ELS];
 
     if(a.skip_if_zero && *a.skip_if_zero == 0.0)
         return;
@@ -314,39 +320,47 @@ __global__ void __launch_bounds__(256) gemm_dma_kernel(const GemmArgsT<double, d
         const int x = bid % 8, l = bid / 8;
         bid         = x * q + min(x, r) + l;
     }
-    constexpr int GROUP = 8;
-    const int     grp   = bid / (GROUP * a.tiles_n);
-    const int     fm    = grp * GROUP;
-    const int     gsz   = min(a.tiles_m - fm, GROUP);
-    const int     m0    = (fm + (bid % gsz)) * 128;
-    const int     n0    = ((bid % (GROUP * a.tiles_n)) / gsz) * 128;
-    const int     ktiles = a.K / 8;
+    constexpr int GROUP  = 8;
+    const int     grp    = bid / (GROUP * a.tiles_n);
+    const int     fm     = grp * GROUP;
+    const int     gsz    = min(a.tiles_m - fm, GROUP);
+    const int     m0     = (fm + (bid % gsz)) * 128;
+    const int     n0     = ((bid % (GROUP * a.tiles_n)) / gsz) * 128;
+    const int     ktiles = a.K / BK;
 
     // this lane's load sources, fixed over K (plus k0 * lda for A, k0 for B)
     const int ai = min(m0 + 2 * lane, a.M - 2);
-    const int jl = lane >> 2;
-    const int kp = (lane & 3) ^ ((jl >> 2) & 3);
-    const int j0 = min(n0 + (2 * w) * 16 + jl, a.N - 1);
-    const int j1 = min(n0 + (2 * w + 1) * 16 + jl, a.N - 1);
+    const int jl = lane / NSLOT;
+    int       bj[PER], bkp[PER];
+#pragma unroll
+    for(int g = 0; g < PER; ++g)
+    {
+        const int jj = (w * PER + g) * (128 / BK) + jl; // row within the tile
+        bj[g]        = min(n0 + jj, a.N - 1);
+        bkp[g]       = (lane % NSLOT) ^ ((jj / JPC) % NSLOT);
+    }
 
     for(int b = blockIdx.y; b < a.batch; b += gridDim.y)
     {
-        const double* Ap  = a.A.get(b);
-        const double* Bp  = a.B.get(b);
-        double*       Cp  = a.C.get(b);
-        const double* sA0 = Ap + ai + int64_t(2 * w) * a.lda;
-        const double* sA1 = Ap + ai + int64_t(2 * w + 1) * a.lda;
-        const double* sB0 = Bp + 2 * kp + int64_t(j0) * a.ldb;
-        const double* sB1 = Bp + 2 * kp + int64_t(j1) * a.ldb;
+        const double* Ap = a.A.get(b);
+        const double* Bp = a.B.get(b);
+        double*       Cp = a.C.get(b);
+        const double* sA = Ap + ai + int64_t(w * PER) * a.lda;
+        const double* sB[PER];
+#pragma unroll
+        for(int g = 0; g < PER; ++g)
+            sB[g] = Bp + 2 * bkp[g] + int64_t(bj[g]) * a.ldb;
 
-        // tiles past the last one re-read the last tile (in bounds) into the
+        // tiles past the last one re-read the last tile (in bounds) into a
         // stage nobody reads any more, so every step waits the same way
-        auto issue = [&](int kt, double* As, double* Bs) {
-            const int64_t k0 = int64_t(min(kt, ktiles - 1)) * 8;
-            dma16(sA0 + k0 * a.lda, As + (2 * w) * DMA_LDA);
-            dma16(sA1 + k0 * a.lda, As + (2 * w + 1) * DMA_LDA);
-            dma16(sB0 + k0, Bs + (2 * w) * 128);
-            dma16(sB1 + k0, Bs + (2 * w + 1) * 128);
+        auto issue = [&](int kt, int st) {
+            const int64_t k0 = int64_t(min(kt, ktiles - 1)) * BK;
+#pragma unroll
+            for(int g = 0; g < PER; ++g)
+                dma16(sA + (k0 + g) * a.lda, &As[st][(w * PER + g) * DMA_LDA]);
+#pragma unroll
+            for(int g = 0; g < PER; ++g)
+                dma16(sB[g] + k0, &Bs[st][(w * PER + g) * 128]);
         };
 
         d4_t acc[4][4];
@@ -356,9 +370,11 @@ __global__ void __launch_bounds__(256) gemm_dma_kernel(const GemmArgsT<double, d
             for(int j = 0; j < 4; ++j)
                 acc[i][j] = d4_t{0, 0, 0, 0};
 
-        auto compute = [&](const double* as, const double* bs) {
+        auto compute = [&](int st) {
+            const double* as = As[st];
+            const double* bs = Bs[st];
 #pragma unroll
-            for(int k4 = 0; k4 < 2; ++k4)
+            for(int k4 = 0; k4 < BK / 4; ++k4)
             {
                 const int kr = k4 * 4 + lane / 16;
                 double    af[4], bf[4];
@@ -369,7 +385,7 @@ __global__ void __launch_bounds__(256) gemm_dma_kernel(const GemmArgsT<double, d
                 for(int j = 0; j < 4; ++j)
                 {
                     const int jj = wn * 64 + j * 16 + (lane & 15);
-                    bf[j] = bs[jj * 8 + ((((kr >> 1) ^ ((jj >> 2) & 3))) << 1) + (kr & 1)];
+                    bf[j] = bs[jj * BK + (((kr >> 1) ^ ((jj / JPC) % NSLOT)) << 1) + (kr & 1)];
                 }
 #pragma unroll
                 for(int i = 0; i < 4; ++i)
@@ -378,29 +394,30 @@ __global__ void __launch_bounds__(256) gemm_dma_kernel(const GemmArgsT<double, d
                         acc[i][j] = Mfma<double>::run(bf[j], af[i], acc[i][j]);
             }
         };
-        // one K step: tiles kt and kt + 1 are in flight (four loads each), so
-        // vmcnt(4) retires tile kt; the barrier publishes it to every
-        // wavefront and retires every read of the stage tile kt + 2 reuses
-        auto step = [&](int kt, const double* as, const double* bs, double* an, double* bn) {
-            wait_vmcnt<4>();
+        // one K step: tiles kt .. kt + NS - 2 are in flight (LOADS each), so
+        // waiting down to (NS - 2) * LOADS retires tile kt; the barrier
+        // publishes it to every wavefront and retires every read of the stage
+        // tile kt + NS - 1 reuses
+        auto step = [&](int kt, int st) {
+            wait_vmcnt<(NS - 2) * LOADS>();
             barrier_no_fence();
-            issue(kt + 2, an, bn);
-            compute(as, bs);
+            issue(kt + NS - 1, (st + NS - 1) % NS);
+            compute(st);
         };
 
         if(ktiles > 0)
         {
-            issue(0, As0, Bs0);
-            issue(1, As1, Bs1);
-            for(int kt = 0; kt < ktiles; kt += 3)
+#pragma unroll
+            for(int t = 0; t < NS - 1; ++t)
+                issue(t, t);
+            for(int kt = 0; kt < ktiles; kt += NS)
             {
-                step(kt, As0, Bs0, As2, Bs2);
-                if(kt + 1 < ktiles)
-                    step(kt + 1, As1, Bs1, As0, Bs0);
-                if(kt + 2 < ktiles)
-                    step(kt + 2, As2, Bs2, As1, Bs1);
+#pragma unroll
+                for(int t = 0; t < NS; ++t)
+                    if(t == 0 || kt + t < ktiles)
+                        step(kt + t, t);
             }
-            wait_vmcnt<0>(); // the two tiles past the end
+            wait_vmcnt<0>(); // the tiles past the end
         }
 
         // epilogue: as gemm_kernel's
@@ -429,11 +446,11 @@ __global__ void __launch_bounds__(256) gemm_dma_kernel(const GemmArgsT<double, d
 }
 
 // The DMA kernel's conditions: plain FP64 operands, 16-byte aligned rows.
-static bool gemm_dma_eligible(const GemmArgsT<double, double, double>& a)
+static bool gemm_dma_eligible(const GemmArgsT<double, double, double>& a, int bk)
 {
     auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-    return a.opA == 0 && a.opB == 0 && !a.tri && a.M >= 2 && a.M % 2 == 0 && a.K >= 8
-           && a.K % 8 == 0 && a.lda % 2 == 0 && a.ldb % 2 == 0 && !a.A.arr && !a.B.arr
+    return a.opA == 0 && a.opB == 0 && !a.tri && a.M >= 2 && a.M % 2 == 0 && a.K >= bk
+           && a.K % bk == 0 && a.lda % 2 == 0 && a.ldb % 2 == 0 && !a.A.arr && !a.B.arr
            && al16(a.A.base + a.A.offset) && al16(a.B.base + a.B.offset) && a.A.stride % 2 == 0
            && a.B.stride % 2 == 0;
 }
@@ -509,14 +526,20 @@ static rocblas_status launch_gemm_cfg(rocblas_handle h, GemmArgsT<TA, TB, TC> a)
     return launch_status();
 }
 
-// ROCBLAS_AMD_GEMM_DMA=0 keeps every GEMM on gemm_kernel (A/B comparisons).
-static bool gemm_dma_disabled()
+// ROCBLAS_AMD_GEMM_DMA (for A/B comparisons): 0 keeps every GEMM on
+// gemm_kernel, 8 uses only the 8-deep, three-stage DMA kernel; by default the
+// 16-deep, two-stage one runs where K allows (8192 x 16384 x 8192 on MI355X:
+// 69.4 TF/s, 8-deep 68.8, gemm_kernel 61.5).
+static int gemm_dma_mode()
 {
-    static const bool off = [] {
+    static const int mode = [] {
         const char* e = std::getenv("ROCBLAS_AMD_GEMM_DMA");
-        return e && e[0] == '0';
+        if(!e)
+            return 16;
+        const int v = std::atoi(e);
+        return v == 0 ? 0 : v == 8 ? 8 : 16;
     }();
-    return off;
+    return mode;
 }
 
 // Pick the largest tile that still gives every CU work.
@@ -542,13 +565,18 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
         if constexpr(std::is_same_v<TA, double> && std::is_same_v<TB, double>
                      && std::is_same_v<TC, double>)
         {
-            if(gemm_dma_eligible(a) && !gemm_dma_disabled())
+            const int mode = gemm_dma_mode();
+            const int bk   = mode == 16 && a.K % 16 == 0 ? 16 : 8;
+            if(mode && gemm_dma_eligible(a, bk))
             {
                 GemmArgsT<TA, TB, TC> d = a;
                 d.tiles_m               = (a.M + 127) / 128;
                 d.tiles_n               = (a.N + 127) / 128;
                 const dim3 grid(unsigned(int64_t(d.tiles_m) * d.tiles_n), grid_batch(a.batch));
-                hipLaunchKernelGGL(gemm_dma_kernel, grid, dim3(256), 0, h->stream, d);
+                if(bk == 16)
+                    hipLaunchKernelGGL((gemm_dma_kernel<16, 2>), grid, dim3(256), 0, h->stream, d);
+                else
+                    hipLaunchKernelGGL((gemm_dma_kernel<8, 3>), grid, dim3(256), 0, h->stream, d);
                 return launch_status();
             }
         }
