@@ -231,22 +231,25 @@ __global__ void __launch_bounds__(WAVE * WGM * WGN)
 
 // ---------------------------------------------------------------------------
 // FP64 GEMM with the operand tiles loaded straight into LDS
-// (global_load_lds_dwordx4): C := alpha * A * B + beta * C, A and B not
-// transposed, dense, 128 x 128 tiles, K steps of BK (8 or 16) and NS LDS
-// stages with NS - 1 tiles in flight.  No staging registers, so the deeper
-// pipeline costs no occupancy.  Each wavefront's load instruction writes
-// 64 x 16 contiguous LDS bytes, lane l at byte 16 l:
-//   A stage: BK rows (k) of 128 doubles (i), row pitch 144 doubles, so the
-//            fragment reads of two k rows fall in disjoint bank halves;
-//            one instruction is one row, lane l holding rows i = 2l, 2l + 1.
-//   B stage: 128 rows (j) of BK doubles (k), 128 / BK rows per instruction;
-//            lane l holds row l / (BK/2) and the k pair
-//            (l % (BK/2)) ^ key(j), an XOR swizzle by j / (256 / (8 BK))
+// (global_load_lds_dwordx4): C := alpha * op(A) * op(B) + beta * C, dense,
+// 128 x 128 tiles, K steps of BK (8 or 16) and NS LDS stages with NS - 1
+// tiles in flight.  No staging registers, so the deeper pipeline costs no
+// occupancy.  Each wavefront's load instruction writes 64 x 16 contiguous LDS
+// bytes, lane l at byte 16 l.  An operand tile (x = row of op(A) or column of
+// op(B), 128 of them, by BK k) takes one of two LDS layouts, by which index
+// is contiguous in memory:
+//   x contiguous (A, or B transposed): BK rows (k) of 128 doubles, pitch 144
+//            doubles, so the fragment reads of two k rows fall in disjoint
+//            bank halves; one instruction is one row, lane l holding
+//            x = 2l, 2l + 1.
+//   k contiguous (B, or A transposed): 128 rows (x) of BK doubles, 128 / BK
+//            rows per instruction; lane l holds row l / (BK/2) and the k
+//            pair (l % (BK/2)) ^ key(x), an XOR swizzle by x / (256 / (8 BK))
 //            that puts the 16 rows a fragment read touches in distinct banks.
-// Eligible when M, lda, ldb and every operand address are even / 16-byte
-// aligned and K is a multiple of BK (launch_gemm checks); rows of A beyond M
-// and columns of B beyond N are read clamped and only feed rows / columns of
-// C that are not stored.
+// Eligible when lda, ldb and every operand address are even / 16-byte
+// aligned, K is a multiple of BK and the extent of an x-contiguous operand
+// (M for A, N for transposed B) is even (launch_gemm checks); rows / columns
+// beyond M / N are read clamped and only feed parts of C that are not stored.
 // ---------------------------------------------------------------------------
 constexpr int DMA_LDA = 144;
 
@@ -289,19 +292,84 @@ __device__ __forceinline__ void barrier_no_fence()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <int BK, int NS>
+// One operand's tiles in the DMA kernel: XC = x contiguous in memory.
+template <int BK, bool XC>
+struct DmaOperand
+{
+    static constexpr int ELS   = XC ? BK * DMA_LDA : 128 * BK;
+    static constexpr int NSLOT = BK / 2;         // k pairs per row (k-contiguous layout)
+    static constexpr int JPC   = 256 / (8 * BK); // rows per 256-byte bank cycle
+    static constexpr int PER   = BK / 4;         // load instructions per wavefront per tile
+
+    // element (k, x) of a stage
+    static __device__ __forceinline__ int at(int k, int x)
+    {
+        if constexpr(XC)
+            return k * DMA_LDA + x;
+        else
+            return x * BK + (((k >> 1) ^ ((x / JPC) % NSLOT)) << 1) + (k & 1);
+    }
+
+    // Per-lane load state, a few 32-bit values so it stays cheap in
+    // registers: the x-contiguous layout keeps this lane's x pair (each
+    // instruction adds a wave-uniform row), the k-contiguous one the row of
+    // its first instruction and its k-pair slot.  Row xl of the tile (for
+    // instruction g of wavefront w: (w PER + g) 128/BK + l / (BK/2)) has the
+    // swizzle key (xl / JPC) % (BK/2) = (4 (w PER + g) + l / 16) % (BK/2).
+    struct Lane
+    {
+        int x;
+        int row0;
+        int slot;
+        int quad;
+    };
+
+    static __device__ __forceinline__ Lane lane_state(int w, int lane, int x0, int X)
+    {
+        Lane l{};
+        if constexpr(XC)
+            l.x = min(x0 + 2 * lane, X - 2);
+        else
+        {
+            l.row0 = x0 + w * PER * (128 / BK) + lane / NSLOT;
+            l.slot = lane % NSLOT;
+            l.quad = lane / 16;
+        }
+        return l;
+    }
+
+    // source of load instruction g of wavefront w at k offset k0
+    static __device__ __forceinline__ const double*
+        src(const double* base, const Lane& l, int g, int w, int64_t k0, int ld, int X)
+    {
+        if constexpr(XC)
+            return base + l.x + (k0 + (w * PER + g)) * ld;
+        else
+        {
+            const int row = min(l.row0 + g * (128 / BK), X - 1);
+            const int key = (4 * (w * PER + g) + l.quad) % NSLOT;
+            return base + (k0 + 2 * (l.slot ^ key)) + int64_t(row) * ld;
+        }
+    }
+
+    // LDS destination of load instruction g of wavefront w
+    static __device__ __forceinline__ int dst(int g, int w)
+    {
+        return XC ? (w * PER + g) * DMA_LDA : (w * PER + g) * 128;
+    }
+};
+
+template <int BK, int NS, int OPA, int OPB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     gemm_dma_kernel(const GemmArgsT<double, double, double> a)
 {
     static_assert((BK == 8 || BK == 16) && NS >= 2 && NS <= 3, "configuration");
-    constexpr int A_ELS = BK * DMA_LDA;
-    constexpr int B_ELS = 128 * BK;
-    constexpr int NSLOT = BK / 2;           // k pairs per B row
-    constexpr int JPC   = 256 / (8 * BK);   // B rows per 256-byte bank cycle
-    constexpr int PER   = BK / 4;           // A rows / B groups per wavefront
-    constexpr int LOADS = 2 * PER;          // loads per wavefront per tile
-    __shared__ __align__(16) double As[NS][A_ELS];
-    __shared__ __align__(16) double Bs[NS][B_ELS];
+    using OA             = DmaOperand<BK, OPA == 0>;
+    using OB             = DmaOperand<BK, OPB == 1>;
+    constexpr int PER    = BK / 4;  // load instructions per operand per wavefront
+    constexpr int LOADS  = 2 * PER; // loads per wavefront per tile
+    __shared__ __align__(16) double As[NS][OA::ELS];
+    __shared__ __align__(16) double Bs[NS][OB::ELS];
 
     if(a.skip_if_zero && *a.skip_if_zero == 0.0)
         return;
@@ -328,28 +396,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     const int     n0     = ((bid % (GROUP * a.tiles_n)) / gsz) * 128;
     const int     ktiles = a.K / BK;
 
-    // this lane's load sources, fixed over K (plus k0 * lda for A, k0 for B)
-    const int ai = min(m0 + 2 * lane, a.M - 2);
-    const int jl = lane / NSLOT;
-    int       bj[PER], bkp[PER];
-#pragma unroll
-    for(int g = 0; g < PER; ++g)
-    {
-        const int jj = (w * PER + g) * (128 / BK) + jl; // row within the tile
-        bj[g]        = min(n0 + jj, a.N - 1);
-        bkp[g]       = (lane % NSLOT) ^ ((jj / JPC) % NSLOT);
-    }
+    const typename OA::Lane la = OA::lane_state(w, lane, m0, a.M);
+    const typename OB::Lane lb = OB::lane_state(w, lane, n0, a.N);
 
     for(int b = blockIdx.y; b < a.batch; b += gridDim.y)
     {
         const double* Ap = a.A.get(b);
         const double* Bp = a.B.get(b);
         double*       Cp = a.C.get(b);
-        const double* sA = Ap + ai + int64_t(w * PER) * a.lda;
-        const double* sB[PER];
-#pragma unroll
-        for(int g = 0; g < PER; ++g)
-            sB[g] = Bp + 2 * bkp[g] + int64_t(bj[g]) * a.ldb;
 
         // tiles past the last one re-read the last tile (in bounds) into a
         // stage nobody reads any more, so every step waits the same way
@@ -357,10 +411,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
             const int64_t k0 = int64_t(min(kt, ktiles - 1)) * BK;
 #pragma unroll
             for(int g = 0; g < PER; ++g)
-                dma16(sA + (k0 + g) * a.lda, &As[st][(w * PER + g) * DMA_LDA]);
+                dma16(OA::src(Ap, la, g, w, k0, a.lda, a.M), &As[st][OA::dst(g, w)]);
 #pragma unroll
             for(int g = 0; g < PER; ++g)
-                dma16(sB[g] + k0, &Bs[st][(w * PER + g) * 128]);
+                dma16(OB::src(Bp, lb, g, w, k0, a.ldb, a.N), &Bs[st][OB::dst(g, w)]);
         };
 
         d4_t acc[4][4];
@@ -380,13 +434,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
                 double    af[4], bf[4];
 #pragma unroll
                 for(int i = 0; i < 4; ++i)
-                    af[i] = as[kr * DMA_LDA + wm * 64 + i * 16 + (lane & 15)];
+                    af[i] = as[OA::at(kr, wm * 64 + i * 16 + (lane & 15))];
 #pragma unroll
                 for(int j = 0; j < 4; ++j)
-                {
-                    const int jj = wn * 64 + j * 16 + (lane & 15);
-                    bf[j] = bs[jj * BK + (((kr >> 1) ^ ((jj / JPC) % NSLOT)) << 1) + (kr & 1)];
-                }
+                    bf[j] = bs[OB::at(kr, wn * 64 + j * 16 + (lane & 15))];
 #pragma unroll
                 for(int i = 0; i < 4; ++i)
 #pragma unroll
@@ -449,8 +500,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 static bool gemm_dma_eligible(const GemmArgsT<double, double, double>& a, int bk)
 {
     auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-    return a.opA == 0 && a.opB == 0 && !a.tri && a.M >= 2 && a.M % 2 == 0 && a.K >= bk
-           && a.K % bk == 0 && a.lda % 2 == 0 && a.ldb % 2 == 0 && !a.A.arr && !a.B.arr
+    // an x-contiguous operand is loaded two rows of op() at a time
+    const bool a_pairs = a.opA == 1 || (a.M >= 2 && a.M % 2 == 0);
+    const bool b_pairs = a.opB == 0 || (a.N >= 2 && a.N % 2 == 0);
+    return !a.tri && a_pairs && b_pairs && a.K >= bk && a.K % bk == 0 && a.lda % 2 == 0
+           && a.ldb % 2 == 0 && !a.A.arr && !a.B.arr
            && al16(a.A.base + a.A.offset) && al16(a.B.base + a.B.offset) && a.A.stride % 2 == 0
            && a.B.stride % 2 == 0;
 }
@@ -526,6 +580,29 @@ static rocblas_status launch_gemm_cfg(rocblas_handle h, GemmArgsT<TA, TB, TC> a)
     return launch_status();
 }
 
+template <int BK, int NS>
+static void launch_gemm_dma_ops(rocblas_handle h, const GemmArgsT<double, double, double>& d,
+                                dim3 grid)
+{
+    if(d.opA == 0 && d.opB == 0)
+        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 0, 0>), grid, dim3(256), 0, h->stream, d);
+    else if(d.opA == 1 && d.opB == 0)
+        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 1, 0>), grid, dim3(256), 0, h->stream, d);
+    else if(d.opA == 0 && d.opB == 1)
+        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 0, 1>), grid, dim3(256), 0, h->stream, d);
+    else
+        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 1, 1>), grid, dim3(256), 0, h->stream, d);
+}
+
+static void launch_gemm_dma(rocblas_handle h, const GemmArgsT<double, double, double>& d,
+                            dim3 grid, int bk)
+{
+    if(bk == 16)
+        launch_gemm_dma_ops<16, 2>(h, d, grid);
+    else
+        launch_gemm_dma_ops<8, 3>(h, d, grid);
+}
+
 // ROCBLAS_AMD_GEMM_DMA (for A/B comparisons): 0 keeps every GEMM on
 // gemm_kernel, 8 uses only the 8-deep, three-stage DMA kernel; by default the
 // 16-deep, two-stage one runs where K allows (8192 x 16384 x 8192 on MI355X:
@@ -573,10 +650,7 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
                 d.tiles_m               = (a.M + 127) / 128;
                 d.tiles_n               = (a.N + 127) / 128;
                 const dim3 grid(unsigned(int64_t(d.tiles_m) * d.tiles_n), grid_batch(a.batch));
-                if(bk == 16)
-                    hipLaunchKernelGGL((gemm_dma_kernel<16, 2>), grid, dim3(256), 0, h->stream, d);
-                else
-                    hipLaunchKernelGGL((gemm_dma_kernel<8, 3>), grid, dim3(256), 0, h->stream, d);
+                launch_gemm_dma(h, d, grid, bk);
                 return launch_status();
             }
         }

@@ -5,7 +5,8 @@ must equal numpy's bit for bit whatever the summation order.  Shapes cover the
 LDS-DMA FP64 kernel (even M, K a multiple of 8, 16-byte aligned operands) at
 tile-aligned and ragged M / N, with beta = 0 and beta != 0, an operand offset
 by one element (misaligned: the register-staged kernel), odd M and K, the
-transposed forms and FP32."""
+transposed forms (the DMA kernel stores a k-contiguous operand row-major with
+a swizzle, an x-contiguous one k-major) and FP32."""
 import numpy as np
 import pytest
 import torch
@@ -41,6 +42,11 @@ CASES = [
     (2048, 2048, 100, "N", "N", 1.0, 0.0, 0, 0, 0, 0),   # K not a multiple of 8
     (2048, 2056, 64, "T", "N", -1.0, 1.0, 0, 0, 0, 0),
     (2048, 2048, 64, "N", "T", 1.0, 1.0, 0, 0, 0, 0),
+    (2050, 2098, 40, "T", "N", 1.0, 1.0, 2, 0, 0, 0),    # K only a multiple of 8
+    (2049, 2048, 64, "T", "N", -1.0, 0.0, 0, 2, 1, 0),   # odd M, A transposed
+    (2048, 2050, 48, "N", "T", 1.0, 1.0, 0, 4, 0, 0),
+    (2052, 2046, 96, "T", "T", -1.0, 1.0, 2, 2, 0, 0),
+    (2048, 2047, 64, "N", "T", 1.0, 0.0, 0, 1, 0, 0),    # odd N, B transposed
 ]
 
 
