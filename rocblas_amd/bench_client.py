@@ -9,7 +9,8 @@ over this library.  A bench line logged with ROCBLAS_LAYER=2 replays as is:
 
 ``--yaml FILE`` times every case of a gtest-format YAML (or its JSON form)
 instead.  Functions: trsm, trsm_batched, trsm_strided_batched and their _ex
-forms, real single and double precision.  ``--api 1`` calls the ILP64 _64
+forms, real single and double precision; and gemm, the update GEMM the
+blocked solve runs on (rocblas_amd_{s,d}gemm).  ``--api 1`` calls the ILP64 _64
 entry points; the Fortran apis call the C entry points (no Fortran module
 is built).
 """
@@ -74,6 +75,69 @@ def parser():
     return p
 
 
+def bench_gemm(handle, ns, out=None):
+    """-f gemm: the library's update GEMM (rocblas_amd_{s,d}gemm, the kernels
+    of the blocked solve's trailing updates) with rocblas-bench's gemm flags,
+    timing loop and result line (testing_gemm.hpp:430-500: transA, transB, M,
+    N, K, alpha, lda, beta, ldb, ldc, then cold/hot iterations, rocblas-Gflops
+    at 2 M N K flop, us; -v 1 adds the CPU figures and the relative 1-norm
+    error against a host FP64 product)."""
+    import time
+
+    import numpy as np
+    import torch
+
+    import rocblas_amd as rb
+
+    prec = C.PRECISIONS.get((ns.compute_type or ns.precision).lower())
+    if prec is None:
+        raise SystemExit("Invalid value for --precision (this library: s, d, f32_r, f64_r)")
+    p = "d" if prec == "f64_r" else "s"
+    dt = torch.float64 if p == "d" else torch.float32
+    ta, tb = ns.transposeA.upper(), ns.transposeB.upper()
+    M, N, K = ns.M, ns.N, ns.K
+    ar, ac = (M, K) if ta == "N" else (K, M)
+    br, bc = (K, N) if tb == "N" else (N, K)
+    lda, ldb, ldc = max(ns.lda, ar), max(ns.ldb, br), max(ns.ldc, M)
+    g = torch.Generator(device="cuda").manual_seed(ns.seed)
+    A = torch.rand(lda * ac, dtype=dt, device="cuda", generator=g) - 0.5
+    B = torch.rand(ldb * bc, dtype=dt, device="cuda", generator=g) - 0.5
+    C0 = torch.rand(ldc * N, dtype=dt, device="cuda", generator=g) - 0.5
+    Cd = C0.clone()
+    call = lambda: rb.gemm(handle, ta, tb, M, N, K, ns.alpha, A.data_ptr(), lda,  # noqa: E731
+                           B.data_ptr(), ldb, ns.beta, Cd.data_ptr(), ldc, precision=p)
+    st = call()
+    if st != rb.rocblas_status.success:
+        raise SystemExit(f"rocblas_amd_{p}gemm: {st.name}")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for i in range(ns.cold_iters + ns.iters):
+        if i == ns.cold_iters:
+            e0.record()
+        call()
+    e1.record()
+    torch.cuda.synchronize()
+    gpu_us = e0.elapsed_time(e1) * 1e3 / max(ns.iters, 1)
+    gflop = 2.0 * M * N * max(K, 1) / 1e9
+    names = "transA,transB,M,N,K,alpha,lda,beta,ldb,ldc,cold_iters,hot_iters,rocblas-Gflops,us"
+    vals = (f"{ta},{tb},{M},{N},{K},{ns.alpha:g},{lda},{ns.beta:g},{ldb},{ldc}, "
+            f"{ns.cold_iters}, {ns.iters}, {gflop / gpu_us * 1e6:g}, {gpu_us:g}")
+    if ns.norm_check or ns.unit_check:
+        f = lambda t, r, c, ld: t.cpu().numpy().astype(np.float64).reshape(c, ld).T[:r]  # noqa
+        opA = f(A, ar, ac, lda) if ta == "N" else f(A, ar, ac, lda).T
+        opB = f(B, br, bc, ldb) if tb == "N" else f(B, br, bc, ldb).T
+        t0 = time.perf_counter()
+        want = ns.alpha * (opA @ opB) + ns.beta * f(C0, M, N, ldc)
+        cpu_us = (time.perf_counter() - t0) * 1e6
+        Cd.copy_(C0)
+        call()
+        torch.cuda.synchronize()
+        err = np.abs(f(Cd, M, N, ldc) - want).sum() / max(np.abs(want).sum(), 1e-300)
+        names += ",CPU-Gflops,CPU-us,norm_error_1"
+        vals += f",{gflop / cpu_us * 1e6:g},{cpu_us:g},{err:g}"
+    print(names + "\n" + vals, file=out or sys.stdout, flush=True)
+    return {"gpu_us": gpu_us}
+
+
 def args_from_cli(ns):
     prec = (ns.compute_type or ns.precision).lower()
     if prec not in C.PRECISIONS:
@@ -128,6 +192,8 @@ def main(argv=None):
                 arg = C.Arguments.from_case(case)
                 arg.timing, arg.unit_check = 1, 0
                 bench_one(h.ptr, arg, log_fn, log_dt)
+        elif ns.function == "gemm":
+            bench_gemm(h, ns)
         else:
             bench_one(h.ptr, args_from_cli(ns), log_fn, log_dt)
     return 0

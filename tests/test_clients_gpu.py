@@ -89,6 +89,21 @@ def test_bench_client_line(capsys):
     assert float(v[14]) > 0
 
 
+def test_bench_client_gemm_line(capsys):
+    """rocblas-bench -f gemm (testing_gemm.hpp's result line) on the update GEMM,
+    with the host check."""
+    rc = bench_client.main("-f gemm -r d --transposeA T --transposeB N -m 640 -n 520 -k 264 "
+                           "--lda 300 --ldb 264 --ldc 640 --alpha -1 --beta 1 -i 2 -j 1 "
+                           "-v 1".split())
+    assert rc == 0
+    names, vals = capsys.readouterr().out.strip().split("\n")[-2:]
+    assert names == ("transA,transB,M,N,K,alpha,lda,beta,ldb,ldc,cold_iters,hot_iters,"
+                     "rocblas-Gflops,us,CPU-Gflops,CPU-us,norm_error_1")
+    v = vals.split(",")
+    assert v[:10] == ["T", "N", "640", "520", "264", "-1", "300", "1", "264", "640"]
+    assert float(v[-1]) < 1e-14
+
+
 CHILD = textwrap.dedent("""
     import ctypes, sys, torch
     sys.path.insert(0, {root!r})
