@@ -24,6 +24,7 @@ import time
 import numpy as np
 
 FP64_PEAK_TF = 78.6
+FP32_PEAK_TF = 157.3
 
 
 def _events(torch, n):
@@ -39,6 +40,7 @@ def vendor_lib(torch):
     vp = ctypes.c_void_p
     V.rocblas_dtrsm.argtypes = [vp] + [ctypes.c_int] * 6 + [vp, vp, ctypes.c_int, vp,
                                                              ctypes.c_int]
+    V.rocblas_strsm.argtypes = V.rocblas_dtrsm.argtypes
     h = vp()
     if V.rocblas_create_handle(ctypes.byref(h)) != 0:
         return None
@@ -156,6 +158,36 @@ def leg_large(torch, rb, oracle, vendor, m=16384, reps=3):
                                              ctypes.c_void_p(dA.data_ptr()), m,
                                              ctypes.c_void_p(dB.data_ptr()), m))
         res["vendor_rocblas_fp64"] = {"ms": t * 1e3, "tflops": flops / t / 1e12}
+    # the same problem in single precision (rocblas_strsm), FP32 copies of A and B
+    dA32, dB032 = dA.float(), dB0.float()
+    dB32 = torch.empty_like(dB032)
+
+    def timed32(fn):
+        ts = []
+        for r in range(reps + 1):
+            dB32.copy_(dB032)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            if r:
+                ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    t = timed32(lambda: rb.check(rb.trsm(h, "L", "U", "N", "N", m, n, 1.0, dA32.data_ptr(), m,
+                                         dB32.data_ptr(), m, precision="s")))
+    e = eta_device(torch, "L", "U", dA32.double().view(m, m).T, dB032.double().view(n, m).T,
+                   dB32.double().view(n, m).T)
+    res["plain_fp32"] = {"ms": t * 1e3, "tflops": flops / t / 1e12,
+                         "frac_fp32_peak": flops / t / 1e12 / FP32_PEAK_TF, "backward_error": e}
+    if vendor is not None:
+        V, vh = vendor
+        one32 = ctypes.c_float(1.0)
+        t = timed32(lambda: V.rocblas_strsm(vh, 141, 121, 111, 131, m, n, ctypes.addressof(one32),
+                                            ctypes.c_void_p(dA32.data_ptr()), m,
+                                            ctypes.c_void_p(dB32.data_ptr()), m))
+        res["vendor_rocblas_fp32"] = {"ms": t * 1e3, "tflops": flops / t / 1e12}
+    del dA32, dB032, dB32
     # CPU oracle on a bounded column sample, scaled to the full width
     A_host = dA.view(m, m).T.cpu().numpy()
     A_host = np.asfortranarray(A_host)
@@ -169,7 +201,8 @@ def leg_large(torch, rb, oracle, vendor, m=16384, reps=3):
     h.close()
     del dA, dB0, dB, A_host
     torch.cuda.empty_cache()
-    return dict({"workload": "dtrsm L/U/N/N m=n=16384, off-diagonal N(0,1)/sqrt(m)"}, **res)
+    return dict({"workload": "dtrsm (and strsm) L/U/N/N m=n=16384, off-diagonal N(0,1)/sqrt(m)"},
+                **res)
 
 
 def leg_variable_batch(torch, rb, vendor, batch=4096, seed=0):

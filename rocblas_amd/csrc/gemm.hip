@@ -496,6 +496,224 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     }
 }
 
+// ---------------------------------------------------------------------------
+// The FP32 form: four-float loads, so the layouts change.
+//   x contiguous: BK rows (k) of 128 floats, two rows per instruction (lanes
+//            0-31, 32-63); lane p of a row holds x quad p ^ 4 (k % 4), which
+//            puts the four k rows a fragment read touches in distinct banks.
+//   k contiguous: 128 rows (x) of BK floats, 64 / (BK/4) rows per
+//            instruction; lane l holds row l / (BK/4) and the k quad
+//            (l % (BK/4)) ^ ((x / (64/BK)) % (BK/4)), distinct banks for the
+//            16 rows of a fragment read.
+// Eligible when lda, ldb and every operand address are multiples of 4 floats
+// / 16 bytes, K is a multiple of BK and the extent of an x-contiguous operand
+// is a multiple of 4.
+// ---------------------------------------------------------------------------
+template <int BK, bool XC>
+struct DmaOperandF
+{
+    static constexpr int ELS = BK * 128;
+    static constexpr int NQ  = BK / 4;                      // k quads per row (k contiguous)
+    static constexpr int RPI = XC ? 2 : 64 / NQ;            // rows per load instruction
+    static constexpr int PER = (XC ? BK : 128) / RPI / 4;   // instructions per wavefront per tile
+    static constexpr int G   = 64 / BK;                     // rows sharing a bank base (k contiguous)
+
+    static __device__ __forceinline__ int at(int k, int x)
+    {
+        if constexpr(XC)
+            return k * 128 + 4 * ((x >> 2) ^ (4 * (k & 3))) + (x & 3);
+        else
+            return x * BK + 4 * ((k >> 2) ^ ((x / G) % NQ)) + (k & 3);
+    }
+
+    struct Lane
+    {
+        int lane;
+        int x0;
+    };
+
+    static __device__ __forceinline__ Lane lane_state(int, int lane, int x0, int)
+    {
+        return Lane{lane, x0};
+    }
+
+    static __device__ __forceinline__ const float*
+        src(const float* base, const Lane& l, int g, int w, int64_t k0, int ld, int X)
+    {
+        const int r0 = (w * PER + g) * RPI; // first row (k for XC, x for k contiguous)
+        if constexpr(XC)
+        {
+            const int r = r0 + l.lane / 32;
+            const int q = (l.lane % 32) ^ (4 * (r & 3));
+            const int x = min(l.x0 + 4 * q, X - 4);
+            return base + x + (k0 + r) * ld;
+        }
+        else
+        {
+            const int xl = r0 + l.lane / NQ;
+            const int kq = (l.lane % NQ) ^ ((xl / G) % NQ);
+            return base + (k0 + 4 * kq) + int64_t(min(l.x0 + xl, X - 1)) * ld;
+        }
+    }
+
+    static __device__ __forceinline__ int dst(int g, int w)
+    {
+        return (w * PER + g) * RPI * (XC ? 128 : BK);
+    }
+};
+
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm" // m0 is reserved; the clobber is still honoured
+__device__ __forceinline__ void dma16(const float* src, const float* lds)
+{
+    const uint32_t dst = __builtin_amdgcn_readfirstlane(uint32_t(
+        reinterpret_cast<uintptr_t>((const __attribute__((address_space(3))) float*)(lds))));
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+                 :
+                 : "v"(src), "s"(dst)
+                 : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+template <int BK, int NS, int OPA, int OPB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    gemm_dma_f32_kernel(const GemmArgsT<float, float, float> a)
+{
+    static_assert((BK == 16 || BK == 32) && NS >= 2 && NS <= 3, "configuration");
+    using OA            = DmaOperandF<BK, OPA == 0>;
+    using OB            = DmaOperandF<BK, OPB == 1>;
+    constexpr int LOADS = OA::PER + OB::PER; // loads per wavefront per tile
+    __shared__ __align__(16) float As[NS][OA::ELS];
+    __shared__ __align__(16) float Bs[NS][OB::ELS];
+
+    if(a.skip_if_zero && *a.skip_if_zero == 0.0f)
+        return;
+
+    const int tid  = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int w    = tid / WAVE;
+    const int wm   = w % 2;
+    const int wn   = w / 2;
+
+    const int total = a.tiles_m * a.tiles_n;
+    int       bid   = blockIdx.x;
+    {
+        const int q = total / 8, r = total % 8;
+        const int x = bid % 8, l = bid / 8;
+        bid         = x * q + min(x, r) + l;
+    }
+    constexpr int GROUP  = 8;
+    const int     grp    = bid / (GROUP * a.tiles_n);
+    const int     fm     = grp * GROUP;
+    const int     gsz    = min(a.tiles_m - fm, GROUP);
+    const int     m0     = (fm + (bid % gsz)) * 128;
+    const int     n0     = ((bid % (GROUP * a.tiles_n)) / gsz) * 128;
+    const int     ktiles = a.K / BK;
+
+    const typename OA::Lane la = OA::lane_state(w, lane, m0, a.M);
+    const typename OB::Lane lb = OB::lane_state(w, lane, n0, a.N);
+
+    for(int b = blockIdx.y; b < a.batch; b += gridDim.y)
+    {
+        const float* Ap = a.A.get(b);
+        const float* Bp = a.B.get(b);
+        float*       Cp = a.C.get(b);
+
+        auto issue = [&](int kt, int st) {
+            const int64_t k0 = int64_t(min(kt, ktiles - 1)) * BK;
+#pragma unroll
+            for(int g = 0; g < OA::PER; ++g)
+                dma16(OA::src(Ap, la, g, w, k0, a.lda, a.M), &As[st][OA::dst(g, w)]);
+#pragma unroll
+            for(int g = 0; g < OB::PER; ++g)
+                dma16(OB::src(Bp, lb, g, w, k0, a.ldb, a.N), &Bs[st][OB::dst(g, w)]);
+        };
+
+        f4_t acc[4][4];
+#pragma unroll
+        for(int i = 0; i < 4; ++i)
+#pragma unroll
+            for(int j = 0; j < 4; ++j)
+                acc[i][j] = f4_t{0, 0, 0, 0};
+
+        auto compute = [&](int st) {
+            const float* as = As[st];
+            const float* bs = Bs[st];
+#pragma unroll
+            for(int k4 = 0; k4 < BK / 4; ++k4)
+            {
+                const int kr = k4 * 4 + lane / 16;
+                float     af[4], bf[4];
+#pragma unroll
+                for(int i = 0; i < 4; ++i)
+                    af[i] = as[OA::at(kr, wm * 64 + i * 16 + (lane & 15))];
+#pragma unroll
+                for(int j = 0; j < 4; ++j)
+                    bf[j] = bs[OB::at(kr, wn * 64 + j * 16 + (lane & 15))];
+#pragma unroll
+                for(int i = 0; i < 4; ++i)
+#pragma unroll
+                    for(int j = 0; j < 4; ++j)
+                        acc[i][j] = Mfma<float>::run(bf[j], af[i], acc[i][j]);
+            }
+        };
+        auto step = [&](int kt, int st) {
+            wait_vmcnt<(NS - 2) * LOADS>();
+            barrier_no_fence();
+            issue(kt + NS - 1, (st + NS - 1) % NS);
+            compute(st);
+        };
+
+        if(ktiles > 0)
+        {
+#pragma unroll
+            for(int t = 0; t < NS - 1; ++t)
+                issue(t, t);
+            for(int kt = 0; kt < ktiles; kt += NS)
+            {
+#pragma unroll
+                for(int t = 0; t < NS; ++t)
+                    if(t == 0 || kt + t < ktiles)
+                        step(kt + t, t);
+            }
+            wait_vmcnt<0>();
+        }
+
+#pragma unroll
+        for(int i = 0; i < 4; ++i)
+        {
+            const int row = m0 + wm * 64 + i * 16 + (lane & 15);
+#pragma unroll
+            for(int j = 0; j < 4; ++j)
+#pragma unroll
+                for(int r = 0; r < 4; ++r)
+                {
+                    const int col = n0 + wn * 64 + j * 16 + Mfma<float>::row(lane, r);
+                    if(row < a.M && col < a.N)
+                    {
+                        float& cref = Cp[row + int64_t(col) * a.ldc];
+                        float  v    = a.alpha * acc[i][j][r];
+                        if(a.beta != 0.0f)
+                            v = fmaf(a.beta, cref, v);
+                        cref = v;
+                    }
+                }
+        }
+        __syncthreads(); // LDS reuse across batch iterations
+    }
+}
+
+static bool gemm_dma_eligible(const GemmArgsT<float, float, float>& a, int bk)
+{
+    auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    // an x-contiguous operand is loaded four rows of op() at a time
+    const bool a_quads = a.opA == 1 || (a.M >= 4 && a.M % 4 == 0);
+    const bool b_quads = a.opB == 0 || (a.N >= 4 && a.N % 4 == 0);
+    return !a.tri && a_quads && b_quads && a.K >= bk && a.K % bk == 0 && a.lda % 4 == 0
+           && a.ldb % 4 == 0 && !a.A.arr && !a.B.arr && al16(a.A.base + a.A.offset)
+           && al16(a.B.base + a.B.offset) && a.A.stride % 4 == 0 && a.B.stride % 4 == 0;
+}
+
 // The DMA kernel's conditions: plain FP64 operands, 16-byte aligned rows.
 static bool gemm_dma_eligible(const GemmArgsT<double, double, double>& a, int bk)
 {
@@ -594,6 +812,29 @@ static void launch_gemm_dma_ops(rocblas_handle h, const GemmArgsT<double, double
         hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 1, 1>), grid, dim3(256), 0, h->stream, d);
 }
 
+template <int BK, int NS>
+static void launch_gemm_dma_f32_ops(rocblas_handle h, const GemmArgsT<float, float, float>& d,
+                                    dim3 grid)
+{
+    if(d.opA == 0 && d.opB == 0)
+        hipLaunchKernelGGL((gemm_dma_f32_kernel<BK, NS, 0, 0>), grid, dim3(256), 0, h->stream, d);
+    else if(d.opA == 1 && d.opB == 0)
+        hipLaunchKernelGGL((gemm_dma_f32_kernel<BK, NS, 1, 0>), grid, dim3(256), 0, h->stream, d);
+    else if(d.opA == 0 && d.opB == 1)
+        hipLaunchKernelGGL((gemm_dma_f32_kernel<BK, NS, 0, 1>), grid, dim3(256), 0, h->stream, d);
+    else
+        hipLaunchKernelGGL((gemm_dma_f32_kernel<BK, NS, 1, 1>), grid, dim3(256), 0, h->stream, d);
+}
+
+static void launch_gemm_dma(rocblas_handle h, const GemmArgsT<float, float, float>& d, dim3 grid,
+                            int bk)
+{
+    if(bk == 32)
+        launch_gemm_dma_f32_ops<32, 2>(h, d, grid);
+    else
+        launch_gemm_dma_f32_ops<16, 3>(h, d, grid);
+}
+
 static void launch_gemm_dma(rocblas_handle h, const GemmArgsT<double, double, double>& d,
                             dim3 grid, int bk)
 {
@@ -604,9 +845,10 @@ static void launch_gemm_dma(rocblas_handle h, const GemmArgsT<double, double, do
 }
 
 // ROCBLAS_AMD_GEMM_DMA (for A/B comparisons): 0 keeps every GEMM on
-// gemm_kernel, 8 uses only the 8-deep, three-stage DMA kernel; by default the
-// 16-deep, two-stage one runs where K allows (8192 x 16384 x 8192 on MI355X:
-// 69.4 TF/s, 8-deep 68.8, gemm_kernel 61.5).
+// gemm_kernel, 8 uses only the shallower DMA kernels (FP64 8-deep with three
+// stages, FP32 16-deep with three stages); by default the deeper two-stage
+// ones run where K allows (8192 x 16384 x 8192 on MI355X: FP64 69.4 TF/s,
+// 8-deep 68.8, gemm_kernel 61.5; FP32 132.4, 16-deep 133, gemm_kernel 106.5).
 static int gemm_dma_mode()
 {
     static const int mode = [] {
@@ -639,6 +881,21 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
     {
         // FP64 runs best with K steps of 8 (two waves per SIMD), FP32 with 16
         // (measured at 8192x16384x8192 on MI355X: 65 vs 50 TF/s, 96 vs 94 TF/s)
+        if constexpr(std::is_same_v<TA, float> && std::is_same_v<TB, float>
+                     && std::is_same_v<TC, float>)
+        {
+            const int mode = gemm_dma_mode();
+            const int bk   = mode != 8 && a.K % 32 == 0 ? 32 : 16;
+            if(mode && gemm_dma_eligible(a, bk))
+            {
+                GemmArgsT<TA, TB, TC> d = a;
+                d.tiles_m               = (a.M + 127) / 128;
+                d.tiles_n               = (a.N + 127) / 128;
+                const dim3 grid(unsigned(int64_t(d.tiles_m) * d.tiles_n), grid_batch(a.batch));
+                launch_gemm_dma(h, d, grid, bk);
+                return launch_status();
+            }
+        }
         if constexpr(std::is_same_v<TA, double> && std::is_same_v<TB, double>
                      && std::is_same_v<TC, double>)
         {

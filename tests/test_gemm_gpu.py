@@ -5,8 +5,10 @@ must equal numpy's bit for bit whatever the summation order.  Shapes cover the
 LDS-DMA FP64 kernel (even M, K a multiple of 8, 16-byte aligned operands) at
 tile-aligned and ragged M / N, with beta = 0 and beta != 0, an operand offset
 by one element (misaligned: the register-staged kernel), odd M and K, the
-transposed forms (the DMA kernel stores a k-contiguous operand row-major with
-a swizzle, an x-contiguous one k-major) and FP32."""
+transposed forms (the DMA kernels store a k-contiguous operand row-major with
+a swizzle, an x-contiguous one k-major) and FP32 (its DMA kernel loads quads:
+shapes with extents and leading dimensions that are or are not multiples of
+4)."""
 import numpy as np
 import pytest
 import torch
@@ -47,6 +49,8 @@ CASES = [
     (2048, 2050, 48, "N", "T", 1.0, 1.0, 0, 4, 0, 0),
     (2052, 2046, 96, "T", "T", -1.0, 1.0, 2, 2, 0, 0),
     (2048, 2047, 64, "N", "T", 1.0, 0.0, 0, 1, 0, 0),    # odd N, B transposed
+    (2052, 2100, 96, "N", "N", -1.0, 1.0, 4, 4, 0, 0),   # ragged tiles, FP32 quads
+    (2048, 2052, 48, "N", "T", 1.0, 1.0, 0, 4, 0, 0),    # K a multiple of 16 only
 ]
 
 
