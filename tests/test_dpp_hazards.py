@@ -7,7 +7,9 @@ VGPR written by a VALU instruction within the two preceding wait states, nor
 follow an EXEC write within five.  This test compiles those translation units
 with the library's own flags to gfx950 assembly and scans each such kernel up
 to its end label: s_nop N counts N+1 wait states, any other instruction one,
-and a label counts as an unknown predecessor.
+and a label counts as an unknown predecessor.  The LDS-DMA GEMM kernels,
+which issue their tile loads as inline assembly, are checked the same way for
+the M0 set-up each load needs.
 """
 import os
 import re
@@ -116,6 +118,33 @@ def test_dpp_kernels_have_no_missing_wait_states(tmp_path):
             problems += [f"{name[:70]} line {ln}: {msg}" for ln, msg in scan(body)]
     assert checked == 3 * 32 + 3 * 2, checked
     assert not problems, "\n".join(problems[:20])
+
+
+@pytest.mark.skipif(not os.path.exists(rbuild.HIPCC), reason="hipcc not available")
+def test_gemm_lds_dma_loads_are_set_up(tmp_path):
+    """The DMA GEMM kernels issue global_load_lds_dwordx4 from inline assembly:
+    every one must follow `s_mov_b32 m0, <scalar register>` (the LDS
+    destination; a vector register there would be wrong) and one wait state
+    (`s_nop 0`), and the kernels must not spill to scratch."""
+    out = compile_asm(os.path.join(rbuild.CSRC, "gemm.hip"), str(tmp_path / "gemm.s"))
+    asm = open(out).read()
+    checked = 0
+    for name, body in kernels(asm, r"gemm_dma"):
+        ops = [l.split(";")[0].strip() for _, l in body]
+        ops = [o for o in ops if o and not o.startswith(".") and not o.endswith(":")]
+        for i, o in enumerate(ops):
+            if o.startswith("global_load_lds"):
+                checked += 1
+                assert ops[i - 1] == "s_nop 0", (name[:60], ops[i - 2:i + 1])
+                m = re.match(r"s_mov_b32 m0, (\S+)$", ops[i - 2])
+                assert m and not re.match(r"v\d|v\[", m.group(1)), (name[:60], ops[i - 2])
+    assert checked >= 16 * 8, checked
+    blocks = re.findall(r"\.amdhsa_kernel (\S+)\n(.*?)\.end_amdhsa_kernel", asm, re.S)
+    dma = [(n, b) for n, b in blocks if "gemm_dma" in n]
+    assert len(dma) == 16, len(dma)
+    for n, b in dma:
+        size = re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", b)
+        assert size and int(size.group(1)) == 0, (n[:60], size and size.group(1))
 
 
 def test_scanner_flags_a_planted_hazard():
