@@ -315,7 +315,12 @@ struct Solver
         return launch_gemm(h, g);
     }
 
-    // SMALL_K < k <= kPanelMaxK: one panel launch solves the whole range
+    // SMALL_K < k <= kPanelMaxK: one launch solves the whole range -- the
+    // chained kernel for one instance whose leaf has at most two vector
+    // panels per 64-row block (a 512-row leaf of 256 vectors: 8 x 16
+    // workgroups instead of 16; with more panels the chain's waits cost more
+    // than the extra workgroups gain: 2048^2 769 vs 499 us), else one
+    // workgroup per panel; both give the same bits
     rocblas_status panel_leaf(int a, int b)
     {
         const int    K  = b - a;
@@ -331,6 +336,9 @@ struct Solver
                   | (p.diag == rocblas_diagonal_unit ? SF_UNIT : 0) | (sideL ? SF_COLS : 0);
         s.alpha_host = T(1);
         s.alpha_dev  = p.alpha_dev; // only the alpha == 0 guard
+        const int blocks = (K + 63) / 64, panels = (s.nvec + 15) / 16;
+        if(panels <= 2 * blocks && chain_fits<T>(h, K, s.nvec, p.batch))
+            return launch_chain<T>(h, s);
         return launch_panel<T>(h, s, p.batch);
     }
 

@@ -443,6 +443,56 @@ def test_chained_single_solve(h, side, uplo, trans):
                                                         .reshape(n, ldb).T), got), b
 
 
+@pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
+def test_recursive_solve_leaves_match_batched(h, side, uplo, trans):
+    """A single instance too large for one chained launch recurses; its
+    leaves (k <= 512) run as chained launches when they have at most two
+    vector panels per 64-row block (12288 x 192: 512-row leaves, 8 x 12
+    workgroups each), else one workgroup per panel (1024 x 1024).  The result
+    meets the oracle's bound (1024) or the backward-error bound (12288, data
+    generated on the device) and equals, bit for bit, the same instance
+    inside a batch of two, whose leaves always take the
+    one-workgroup-per-panel kernel."""
+    rng = np.random.default_rng(910)
+    for (k, nv) in [(1024, 1024), (12288, 192)]:
+        m, n = (k, nv) if side == "L" else (nv, k)
+        say("recursive leaves", side, uplo, trans, m, n)
+        if k <= 2048:
+            A = oracle.diag_dominant_triangular(k, uplo, rng, lda=k)
+            B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+            st, got = gpu_trsm(h, side, uplo, trans, "N", m, n, -0.5, A, B)
+            assert st == S.success
+            ref = B.copy(order="F")
+            oracle.trsm(side, uplo, trans, "N", m, n, -0.5, A, ref)
+            assert oracle.norm1_rel_diff(m, n, got, ref) <= 40 * EPS64 * k
+            dA1 = torch.from_numpy(A.ravel(order="F").copy()).cuda()
+            dB1 = torch.from_numpy(B.ravel(order="F").copy()).cuda()
+            X1 = torch.from_numpy(np.ascontiguousarray(got.ravel(order="F"))).cuda()
+        else:
+            g = torch.Generator(device="cuda").manual_seed(k + nv)
+            dA1 = torch.randn(k * k, dtype=torch.float64, device="cuda", generator=g) / k ** 0.5
+            dA1.view(k, k).diagonal().add_(2.0)
+            dB1 = torch.rand(m * n, dtype=torch.float64, device="cuda", generator=g) - 0.5
+            X1 = dB1.clone()
+            st = rb.trsm(h, side, uplo, trans, "N", m, n, -0.5, dA1.data_ptr(), k,
+                         X1.data_ptr(), m)
+            assert st == S.success
+            torch.cuda.synchronize()
+            Al = dA1.view(k, k).T
+            eta = eta_torch(side, uplo, trans, "N", -0.5, Al, dB1.view(n, m).T, X1.view(n, m).T)
+            assert eta <= 1e-14, eta
+        sa, sb = k * k, m * n
+        dA = torch.cat([dA1, dA1])
+        dB = torch.cat([dB1, dB1])
+        st = rb.trsm_strided_batched(h, side, uplo, trans, "N", m, n, -0.5, dA.data_ptr(), k, sa,
+                                     dB.data_ptr(), m, sb, 2)
+        assert st == S.success
+        torch.cuda.synchronize()
+        for b in range(2):
+            assert torch.equal(dB[b * sb:(b + 1) * sb], X1), b
+        del dA, dB, dA1, dB1, X1
+
+
 # ------------------------------------------------------------ trsm_ex --
 
 def test_trsm_ex_forms_match_typed_entry_points(h):
