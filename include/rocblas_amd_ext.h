@@ -260,6 +260,24 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb
                                                                   rocblas_int waves_per_cu,
                                                                   rocblas_int panel_groups);
 
+/* Completes the handle's stream (skipped while the stream is being captured)
+ * and reports whether a kernel of an earlier call on this handle gave up one
+ * of its bounded waits, which leaves that call's result wrong:
+ * rocblas_status_internal_error if so (the report is then cleared), else
+ * success.  Only the chained single-instance solve waits (on other workgroups
+ * of its own launch, bounded to about a second).  The next call on the handle
+ * that launches a solve also reports and clears it, returning
+ * rocblas_status_internal_error without doing any work.  No counterpart in
+ * the reference, whose waits are unbounded. */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_get_async_status(rocblas_handle handle);
+
+/* Test hook for the failure path above: with enable != 0, every chained
+ * launch on this handle gives up each wait after a few polls (its result is
+ * wrong and the error is raised; every kernel still runs to its end).  0
+ * restores normal launches.  No counterpart in the reference. */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_set_chain_test_timeout(rocblas_handle handle,
+                                                                     int            enable);
+
 #ifdef __cplusplus
 }
 #endif

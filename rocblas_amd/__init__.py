@@ -158,6 +158,8 @@ SYMBOLS = [
                                       ctypes.c_double, _vp, _i32]),
     ("rocblas_amd_dsynth_rhs", _i32, [_vp, _i32, _i32, ctypes.c_uint64, _i32, _vp, _i32]),
     ("rocblas_amd_ssynth_rhs", _i32, [_vp, _i32, _i32, ctypes.c_uint64, _i32, _vp, _i32]),
+    ("rocblas_amd_get_async_status", _i32, [_vp]),
+    ("rocblas_amd_set_chain_test_timeout", _i32, [_vp, _i32]),
 ]
 
 
@@ -220,6 +222,14 @@ class Handle:
 
     def set_pointer_mode(self, mode):
         return lib().rocblas_set_pointer_mode(self.ptr, int(mode))
+
+    def get_async_status(self):
+        """rocblas_amd_get_async_status: completes the stream, reports (and clears)
+        a wait that a kernel of an earlier call gave up."""
+        return rocblas_status(lib().rocblas_amd_get_async_status(self.ptr))
+
+    def set_chain_test_timeout(self, enable):
+        return rocblas_status(lib().rocblas_amd_set_chain_test_timeout(self.ptr, int(enable)))
 
 
 def trsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, precision="d",

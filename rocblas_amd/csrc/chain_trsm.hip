@@ -20,11 +20,17 @@
 //
 // Ordering: workgroup (p, I) has linear index I * P + p and only waits for
 // smaller linear indices, which are dispatched first, so the chain cannot
-// deadlock.  Every wait is bounded (kChainSpinLimit polls, about a second): a
-// wait that runs out sets the sticky error word after the counters and
-// proceeds, so every wave reaches its exit.  The last block row resets its
-// panel's counter to zero, leaving the counters ready for the next launch
-// (also a graph replay).
+// deadlock.  Every wait is bounded (about a second of polls): a wait that runs
+// out raises the handle's error word (host-visible memory, read by the next
+// call on the handle and by rocblas_amd_get_async_status) and proceeds, so
+// every wave reaches its exit.
+//
+// Counter word of panel p: the low 32 bits count the block rows that have
+// published, the high 32 bits the workgroups that have finished.  Every
+// workgroup ends with one atomic add of (1 << 32) + 1; the one that sees the
+// other R - 1 finishes in the value it replaced is the last user of the word
+// and stores zero.  So the counters are zero after every launch -- after a
+// wait that gave up, too -- and a graph replay starts from zero.
 //
 // Visibility across the XCDs' separate L2 caches: each wave ends its stores
 // of y_I with an agent-scope release fence (L2 write-back, wait for the
@@ -39,11 +45,24 @@ namespace
 {
 using namespace panel;
 
+// Polls of one wait before it gives up (about a second with s_sleep 2).
 constexpr int kChainSpinLimit = 1 << 20;
+constexpr unsigned long long kRowsMask = 0xffffffffull;
+constexpr unsigned long long kFinishOne = 1ull << 32;
+
+// Wait controls: the poll limit and an offset added to every wait's target.
+// Normal launches use kChainSpinLimit and 0; a test hook of the handle sets a
+// small limit and an unreachable offset so that every wait gives up.
+struct ChainWait
+{
+    int      spin_limit;
+    unsigned need_bias;
+};
 
 template <typename T>
 __global__ void __launch_bounds__(PNT)
-    chain_trsm_kernel(const SmallArgs<T> a, unsigned long long* cnt, int P, int R)
+    chain_trsm_kernel(const SmallArgs<T> a, unsigned long long* cnt, unsigned int* err, int P,
+                      int R, ChainWait cw)
 {
     extern __shared__ __align__(16) unsigned char chain_smem[];
     T* const Ls = reinterpret_cast<T*>(chain_smem); // PNB x PLDA staged tile L_IJ
@@ -51,9 +70,7 @@ __global__ void __launch_bounds__(PNT)
     T* const Cs = Ln + PNB * PLDA;                  // PNB x PNC accumulated update
     T* const dg = Cs + PNB * PNC;                   // PNB diagonal of L_II
     T* const Yj = dg + PNB;                         // PNB x PLDY solved block y_J
-    __shared__ unsigned long long seen;             // last counter value observed
-
-    unsigned long long* const err = cnt + P;
+    __shared__ unsigned long long seen;             // rows seen published (low word)
 
     const int  I    = blockIdx.x / P;
     const int  p    = blockIdx.x % P;
@@ -155,9 +172,10 @@ __global__ void __launch_bounds__(PNT)
     // counter[p] >= need: bounded poll by one thread, then every wave acquires
     if(tid == 0)
         seen = 0;
-    auto wait_rows = [&](int need) {
+    auto wait_rows = [&](int need_rows) {
+        const unsigned long long need = (unsigned long long)need_rows + cw.need_bias;
         __syncthreads();
-        if(seen >= (unsigned long long)need)
+        if(seen >= need)
             return;
         __syncthreads(); // every thread has read `seen` before it changes
         if(tid == 0)
@@ -165,13 +183,14 @@ __global__ void __launch_bounds__(PNT)
             unsigned long long v = 0;
             for(int it = 0;; ++it)
             {
-                v = __hip_atomic_load(cnt + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if(v >= (unsigned long long)need)
+                v = __hip_atomic_load(cnt + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    & kRowsMask;
+                if(v >= need)
                     break;
-                if(it >= kChainSpinLimit)
+                if(it >= cw.spin_limit)
                 {
-                    __hip_atomic_store(err, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    v = ~0ull; // give up: every later wait of this workgroup returns
+                    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    v = kRowsMask; // give up: every later wait of this workgroup returns
                     break;
                 }
                 __builtin_amdgcn_s_sleep(2);
@@ -268,19 +287,16 @@ __global__ void __launch_bounds__(PNT)
             gstore(Bp + boff(gi, vec), c[r] * rd[r]);
     }
 
-    // ---- publish y_I ----
-    if(I + 1 < R)
+    // ---- publish y_I and count this workgroup finished ----
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); // this wave's stores, written back
+    __syncthreads();
+    if(tid == 0)
     {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); // this wave's stores, written back
-        __syncthreads();
-        if(tid == 0)
-            __hip_atomic_store(cnt + p, (unsigned long long)(I + 1), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    }
-    else if(tid == 0 && I > 0)
-    {
-        // every other block row of this panel has finished waiting
-        __hip_atomic_store(cnt + p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long old = __hip_atomic_fetch_add(
+            cnt + p, kFinishOne + 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        // the last of the panel's R workgroups: nobody reads the word any more
+        if((old >> 32) == (unsigned long long)(R - 1))
+            __hip_atomic_store(cnt + p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 } // namespace
@@ -311,9 +327,13 @@ rocblas_status launch_chain(rocblas_handle h, const SmallArgs<T>& a)
     if(!chain_fits<T>(h, a.K, a.nvec, 1))
         return rocblas_status_internal_error;
     const int R = (a.K + PNB - 1) / PNB, P = (a.nvec + PNC - 1) / PNC;
+    ChainWait cw{kChainSpinLimit, 0u};
+    if(h->chain_debug_timeout)
+        cw = ChainWait{64, 1u << 30};
     hipLaunchKernelGGL(chain_trsm_kernel<T>, dim3(unsigned(R * P)), dim3(PNT),
                        chain_lds_bytes(sizeof(T)), h->stream, a,
-                       static_cast<unsigned long long*>(h->chain_flags), P, R);
+                       static_cast<unsigned long long*>(h->chain_flags), h->async_error, P, R,
+                       cw);
     return launch_status();
 }
 

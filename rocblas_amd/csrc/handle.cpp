@@ -3,6 +3,8 @@
 // library/src/rocblas_auxiliary.cpp and library/src/handle.cpp (cited per
 // function); the implementation is this library's own.
 #include "internal.hpp"
+
+#include "../../include/rocblas_amd_ext.h"
 #include "logging.hpp"
 
 #include <cstdlib>
@@ -138,11 +140,17 @@ try
         }
         h->ws_size = want;
     }
-    // completion counters of the chained solve, zero between launches
+    // completion counters of the chained solve, zero between launches, and
+    // the host-visible error word its bounded waits raise
     if(hipMalloc(&h->chain_flags, rbamd::kChainFlagsBytes) != hipSuccess
        || hipMemset(h->chain_flags, 0, rbamd::kChainFlagsBytes) != hipSuccess
+       || hipHostMalloc(reinterpret_cast<void**>(&h->async_error), sizeof(unsigned int),
+                        hipHostMallocCoherent)
+              != hipSuccess
        || hipDeviceSynchronize() != hipSuccess)
     {
+        if(h->async_error)
+            (void)hipHostFree(h->async_error);
         if(h->chain_flags)
             (void)hipFree(h->chain_flags);
         if(h->ws)
@@ -150,6 +158,7 @@ try
         delete h;
         return rocblas_status_memory_error;
     }
+    *h->async_error = 0u;
     rbamd::init_logging(h);
     *handle = h;
     return rocblas_status_success;
@@ -175,6 +184,8 @@ rocblas_status rocblas_destroy_handle(rocblas_handle handle)
         (void)hipDeviceSynchronize();
         (void)hipFree(handle->chain_flags);
     }
+    if(handle->async_error)
+        (void)hipHostFree(handle->async_error);
     delete handle;
     return rocblas_status_success;
 }
@@ -362,4 +373,31 @@ bool rocblas_is_user_managing_device_memory(rocblas_handle handle)
     return handle && handle->owner == WorkspaceOwner::user_managed;
 }
 
+// No counterpart in the reference.  Completes the handle's stream (unless it is
+// being captured) and reports whether a kernel of an earlier call gave up a
+// bounded wait, clearing the report.
+rocblas_status rocblas_amd_get_async_status(rocblas_handle handle)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if(handle->stream)
+        (void)hipStreamIsCapturing(handle->stream, &cs);
+    if(cs == hipStreamCaptureStatusNone)
+    {
+        rbamd::DeviceGuard dg(handle->device);
+        hipError_t e = hipStreamSynchronize(handle->stream);
+        if(e != hipSuccess)
+            return rbamd::hip_to_status(e);
+    }
+    return rbamd::take_async_error(handle);
+}
+
+rocblas_status rocblas_amd_set_chain_test_timeout(rocblas_handle handle, int enable)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    handle->chain_debug_timeout = enable != 0;
+    return rocblas_status_success;
+}
 } // extern "C"

@@ -41,8 +41,14 @@ struct _rocblas_handle
     bool                 size_query    = false;
     size_t               query_size    = 0;
     // completion counters of the chained solve (chain_trsm.hip): one per
-    // panel, then a sticky error word; zero between launches
+    // panel, zero between launches
     void*                chain_flags   = nullptr;
+    // host-visible word a kernel raises when one of its bounded waits gave
+    // up; read and cleared by the next launching call and by
+    // rocblas_amd_get_async_status
+    unsigned int*        async_error   = nullptr;
+    // test hook: chained launches give up every wait after a few polls
+    bool                 chain_debug_timeout = false;
     // ROCBLAS_LAYER bits read at creation and the streams they opened
     int                  layer_mode     = 0;
     rbamd::LogSink*      log_trace_os   = nullptr;
@@ -67,6 +73,18 @@ inline size_t roundup_ws(size_t s)
 constexpr size_t kDefaultWorkspace = size_t(128) << 20;
 
 rocblas_status hip_to_status(hipError_t e);
+
+// A wait of an earlier launch on this handle gave up (its result is wrong):
+// clears the word and returns rocblas_status_internal_error, else success.
+inline rocblas_status take_async_error(rocblas_handle h)
+{
+    if(h->async_error && __atomic_load_n(h->async_error, __ATOMIC_ACQUIRE) != 0u)
+    {
+        __atomic_store_n(h->async_error, 0u, __ATOMIC_RELEASE);
+        return rocblas_status_internal_error;
+    }
+    return rocblas_status_success;
+}
 
 // Temporarily make the handle's device current (restored on destruction).
 struct DeviceGuard

@@ -375,6 +375,9 @@ size_t trsm_workspace_bytes(const TrsmProblem<T>&)
 template <typename T>
 rocblas_status trsm_launch(rocblas_handle h, const TrsmProblem<T>& p)
 {
+    // a chained launch of an earlier call gave up a wait: report it here
+    if(rocblas_status st = take_async_error(h); st != rocblas_status_success)
+        return st;
     DeviceGuard dg(h->device);
     const bool  sideL  = p.side == rocblas_side_left;
     const bool  tA     = sideL ? (p.trans != rocblas_operation_none)

@@ -309,3 +309,131 @@ def test_trtri_against_numpy_and_exact(uplo, diag):
     oracle.trtri(uplo, "N", n, got)
     want = np.tril(np.ones((n, n))) if uplo == "L" else np.triu(np.ones((n, n)))
     assert np.array_equal(got, want)
+
+
+def test_norm1_rel_diff_planted_differences():
+    """||X - Y||_1 / ||Y||_1 (clients/include/norm.hpp:449-492) on planted
+    differences with closed-form answers.  Y's column sums (2m, m, m) differ from
+    its row sums (n + 1), so a row/column swap fails; two differences in
+    different columns give the larger one (max over columns, not a total); two
+    in one column add; rows past m (NaN padding) are never read."""
+    m, n = 5, 3
+    Y = np.full((m + 2, n), np.nan, order="F")
+    Y[:m, :] = 1.0
+    Y[:m, 0] = 2.0
+    X = Y.copy(order="F")
+    assert oracle.norm1_rel_diff(m, n, X, Y) == 0.0
+    X[3, 1] += 0.25
+    assert oracle.norm1_rel_diff(m, n, X, Y) == 0.25 / (2 * m)
+    X[0, 2] -= 0.5
+    assert oracle.norm1_rel_diff(m, n, X, Y) == 0.5 / (2 * m)
+    X[4, 2] += 0.125
+    assert oracle.norm1_rel_diff(m, n, X, Y) == 0.625 / (2 * m)
+    # a zero reference: 0 for equal inputs, infinity otherwise
+    Z = np.zeros((m, n), order="F")
+    assert oracle.norm1_rel_diff(m, n, Z, Z) == 0.0
+    assert oracle.norm1_rel_diff(m, n, np.asfortranarray(Z + 1.0), Z) == np.inf
+    # an empty block is 0
+    assert oracle.norm1_rel_diff(0, n, X, Y) == 0.0
+
+
+@pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_float32_integer_kat_bitwise(side, uplo, trans, diag):
+    """The single-precision loop nest on exactly solvable problems: every
+    intermediate is a small integer or dyadic, exact in binary32, so the solve
+    reproduces X bit for bit."""
+    rng = np.random.default_rng(37)
+    for (m, n) in [(1, 1), (6, 4), (19, 11), (40, 33)]:
+        for alpha in (1.0, -0.5, 4.0):
+            A, B, X = integer_kat(side, uplo, trans, diag, m, n, rng,
+                                  lda=(m if side == "L" else n) + 2, ldb=m + 3, alpha=alpha)
+            A32 = np.asfortranarray(A.astype(np.float32))
+            B32 = np.asfortranarray(B.astype(np.float32))
+            assert np.array_equal(B32.astype(np.float64)[:m, :n], B[:m, :n])
+            oracle.trsm(side, uplo, trans, diag, m, n, alpha, A32, B32)
+            assert B32.dtype == np.float32
+            assert np.array_equal(B32[:m, :n].astype(np.float64), X[:m, :n]), (m, n, alpha)
+            assert np.all(B32[m:, :] == 0)  # rows past m untouched
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_trtri_float32_exact_and_numpy(uplo):
+    """Single-precision xTRTI2: the bidiagonal I - N has the all-ones triangle as
+    its exact inverse; with a power-of-two diagonal D the inverse of D - N has
+    entries d_j^{-1} products, all dyadic and exact in binary32; a random
+    diagonally dominant triangle matches numpy's float64 inverse to float32
+    accuracy."""
+    n = 24
+    A = np.zeros((n + 1, n), dtype=np.float32, order="F")
+    A[np.arange(n), np.arange(n)] = 1.0
+    for i in range(n - 1):
+        if uplo == "L":
+            A[i + 1, i] = -1.0
+        else:
+            A[i, i + 1] = -1.0
+    got = A.copy(order="F")
+    oracle.trtri(uplo, "N", n, got)
+    want = np.tril(np.ones((n, n))) if uplo == "L" else np.triu(np.ones((n, n)))
+    assert got.dtype == np.float32
+    assert np.array_equal(got[:n, :n].astype(np.float64), want)
+    # diag 2: inverse entries are 2^-(distance+1)
+    A2 = A.copy(order="F")
+    A2[np.arange(n), np.arange(n)] = 2.0
+    got2 = A2.copy(order="F")
+    oracle.trtri(uplo, "N", n, got2)
+    i, j = np.indices((n, n))
+    dist = (i - j) if uplo == "L" else (j - i)
+    want2 = np.where(dist >= 0, 2.0 ** (-(dist.astype(float) + 1)), 0.0)
+    assert np.array_equal(got2[:n, :n].astype(np.float64), want2)
+    # unit diagonal: the diagonal is never read (NaN) and the inverse of I - N again
+    A3 = A.copy(order="F")
+    A3[np.arange(n), np.arange(n)] = np.nan
+    got3 = A3.copy(order="F")
+    oracle.trtri(uplo, "U", n, got3)
+    off = (dist > 0)
+    assert np.array_equal(got3[:n, :n][off].astype(np.float64), want[off])
+    assert np.all(np.isnan(np.diag(got3[:n, :n])))
+    rng = np.random.default_rng(41)
+    for k in (3, 17, 50):
+        R = oracle.diag_dominant_triangular(k, uplo, rng, dtype=np.float32)
+        T = op_dense(R, uplo, "N", "N", k)
+        inv = op_dense(oracle.trtri(uplo, "N", k, R.copy(order="F")), uplo, "N", "N", k)
+        ref = np.linalg.inv(T)
+        assert np.abs(inv - ref).max() <= 64 * EPS32 * np.abs(ref).max()
+
+
+def test_backward_error_float32_closed_forms():
+    """Single-precision eta: a planted error against A = I (unit diagonal, never
+    read) with every quantity exact, so eta has a closed form; the float32 solve
+    gives eta at binary32 rounding level, and the wrong side or transpose does
+    not."""
+    m, n = 12, 7
+    A = np.zeros((m, m), dtype=np.float32, order="F")
+    np.fill_diagonal(A, np.nan)
+    B0 = np.full((m, n), 0.25, dtype=np.float32, order="F")
+    X = np.full((m, n), 0.5, dtype=np.float32, order="F")
+    X[3, 2] += np.float32(2.0 ** -12)
+    want = 2.0 ** -12 / (m * 0.5 + 2.0 ** -12 + 2.0 * m * 0.25)
+    got = oracle.backward_error("L", "L", "N", "U", m, n, 2.0, A, B0, X)
+    assert got == pytest.approx(want, rel=1e-15)
+    X[3, 2] = 0.5
+    assert oracle.backward_error("L", "L", "N", "U", m, n, 2.0, A, B0, X) == 0.0
+    rng = np.random.default_rng(43)
+    A = oracle.diag_dominant_triangular(m, "U", rng, dtype=np.float32)
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(np.float32))
+    X = B0.copy(order="F")
+    oracle.trsm("L", "U", "T", "N", m, n, 1.5, A, X)
+    assert oracle.backward_error("L", "U", "T", "N", m, n, 1.5, A, B0, X) <= 8 * EPS32
+    assert oracle.backward_error("L", "U", "N", "N", m, n, 1.5, A, B0, X) > 1e-3
+    B0r = np.asfortranarray(rng.uniform(-1, 1, (n, m)).astype(np.float32))
+    Xr = B0r.copy(order="F")
+    oracle.trsm("R", "U", "T", "N", n, m, 1.5, A, Xr)
+    assert oracle.backward_error("R", "U", "T", "N", n, m, 1.5, A, B0r, Xr) <= 8 * EPS32
+    assert oracle.backward_error("R", "U", "N", "N", n, m, 1.5, A, B0r, Xr) > 1e-3
+    # the float32 value equals the float64 routine on the same (exactly widened) data
+    e64 = oracle.backward_error("R", "U", "T", "N", n, m, 1.5, np.asfortranarray(A.astype(float)),
+                                np.asfortranarray(B0r.astype(float)),
+                                np.asfortranarray(Xr.astype(float)))
+    assert oracle.backward_error("R", "U", "T", "N", n, m, 1.5, A, B0r, Xr) == pytest.approx(
+        e64, rel=1e-12)

@@ -348,6 +348,104 @@ def test_graph_capture_replays_match_eager(h):
     hg.close()
 
 
+@pytest.mark.parametrize("case", ["chained", "recursive", "device_alpha"])
+def test_graph_capture_multi_launch_chained_device_mode(h, case):
+    """Graph capture of the paths the one-launch test above does not reach: the
+    chained solve (its counters must be back at zero for every replay), the
+    recursive path (scale, GEMM updates and leaf launches), and device
+    pointer mode (alpha read on the device, no synchronisation).  Every replay,
+    with eager work between replays, equals the eager result bit for bit."""
+    rng = np.random.default_rng(811)
+    m, n, uplo, trans = {"chained": (512, 256, "L", "N"), "recursive": (1500, 700, "U", "T"),
+                         "device_alpha": (700, 40, "L", "T")}[case]
+    say("graph", case, m, n)
+    A = synth.tri_uniform(m, uplo, 5)
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    _, want = gpu_trsm(h, "L", uplo, trans, "N", m, n, -1.5, A, B0)
+    dA, dB0 = dev(A), dev(B0)
+    dB = dB0.clone()
+    s = torch.cuda.Stream()
+    hg = GpuHandle()
+    rb.check(hg.set_stream(s.cuda_stream))
+    alpha_d = torch.full((1,), -1.5, dtype=torch.float64, device="cuda")
+    if case == "device_alpha":
+        rb.check(hg.set_pointer_mode(rb.rocblas_pointer_mode.device))
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        if case == "device_alpha":
+            st = rb.trsm(hg, "L", uplo, trans, "N", m, n, None, dA.data_ptr(), m, dB.data_ptr(),
+                         m, alpha_ptr=alpha_d.data_ptr())
+        else:
+            st = rb.trsm(hg, "L", uplo, trans, "N", m, n, -1.5, dA.data_ptr(), m, dB.data_ptr(),
+                         m)
+    assert st == S.success
+    for rep in range(4):
+        dB.copy_(dB0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(dB, (m, n)), want), rep
+        _ = (dB0 * 2).abs().sum().item()
+        if case == "chained":
+            # an eager chained solve on the same handle between replays
+            dE = dB0.clone()
+            assert rb.trsm(hg, "L", uplo, trans, "N", m, n, -1.5, dA.data_ptr(), m,
+                           dE.data_ptr(), m) == S.success
+            torch.cuda.synchronize()
+            assert np.array_equal(host(dE, (m, n)), want)
+    assert hg.get_async_status() == S.success
+    hg.close()
+
+
+def test_chained_wait_timeout_is_reported_and_heals(h):
+    """A chained launch whose waits give up (forced from the host: a few polls
+    and an unreachable target, so every kernel still runs to its end) leaves a
+    wrong result; the handle reports it through rocblas_amd_get_async_status
+    and through the next launching call, which does no work; the counters are
+    back at zero, so the following calls -- with more vector panels than the
+    failed launch used -- give the normal result."""
+    rng = np.random.default_rng(812)
+    m, n = 512, 256
+    A = synth.tri_uniform(m, "L", 8)
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (m, n + 512)))
+    _, want = gpu_trsm(h, "L", "L", "N", "N", m, n, 1.0, A, B0[:, :n].copy(order="F"))
+    _, want_wide = gpu_trsm(h, "L", "L", "N", "N", m, n + 512, 1.0, A, B0)
+    dA = dev(A)
+    ht = GpuHandle()
+    try:
+        # 1: reported by the query, once
+        assert ht.set_chain_test_timeout(1) == S.success
+        dB = dev(B0[:, :n].copy(order="F"))
+        say("chain timeout forced (query)")
+        assert rb.trsm(ht, "L", "L", "N", "N", m, n, 1.0, dA.data_ptr(), m, dB.data_ptr(), m)             == S.success
+        assert ht.get_async_status() == S.internal_error
+        assert ht.get_async_status() == S.success
+        # 2: reported by the next launching call, which leaves B alone
+        dB = dev(B0[:, :n].copy(order="F"))
+        say("chain timeout forced (next call)")
+        assert rb.trsm(ht, "L", "L", "N", "N", m, n, 1.0, dA.data_ptr(), m, dB.data_ptr(), m)             == S.success
+        torch.cuda.synchronize()
+        assert ht.set_chain_test_timeout(0) == S.success
+        dC = dev(B0[:, :n].copy(order="F"))
+        assert rb.trsm(ht, "L", "L", "N", "N", m, n, 1.0, dA.data_ptr(), m, dC.data_ptr(), m)             == S.internal_error
+        torch.cuda.synchronize()
+        assert np.array_equal(host(dC, (m, n)), B0[:, :n])
+        # 3: healed -- same width, then more panels than the failed launch had
+        say("chain after timeout")
+        assert rb.trsm(ht, "L", "L", "N", "N", m, n, 1.0, dA.data_ptr(), m, dC.data_ptr(), m)             == S.success
+        torch.cuda.synchronize()
+        assert np.array_equal(host(dC, (m, n)), want)
+        dW = dev(B0)
+        assert rb.trsm(ht, "L", "L", "N", "N", m, n + 512, 1.0, dA.data_ptr(), m, dW.data_ptr(),
+                       m) == S.success
+        torch.cuda.synchronize()
+        assert np.array_equal(host(dW, (m, n + 512)), want_wide)
+        assert ht.get_async_status() == S.success
+    finally:
+        ht.close()
+
+
 # --------------------------------------------------------- workload sizes --
 
 def test_batched_workload_size_all_cases(h):
