@@ -188,16 +188,24 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_ssynth_rhs(rocblas_handle handle,
 /* ---- serving loop ---------------------------------------------------------
  * A server owns one device arena of hbm_budget_bytes (its admission budget)
  * and `streams` streams with a handle each.  rocblas_amd_server_run replays a
- * trace of requests, each the solve L X = B (side left, lower, no transpose,
- * non-unit, alpha 1) with a synthetic factor L (uniform distribution of
- * rocblas_amd_{s,d}synth_tri, kept resident per factor_id and evicted least
- * recently used when the arena is full) and a synthetic B: requests are
- * admitted FIFO once they have arrived and their memory fits, up to
- * max_inflight at once; those with m <= small_k are solved together in one
- * _vbatched launch per precision on the first stream, larger ones by
- * rocblas_{s,d}trsm over the other streams in turn, so small requests never
- * wait behind large solves and new work overlaps work in flight.  latency[i] is the
- * host time from arrival to observed completion (-1 if rejected).  No
+ * trace of requests.  Request i solves op(A) X = alpha B (side left) or
+ * X op(A) = alpha B (side right) with its own side, uplo, transA, diag and
+ * alpha, a synthetic triangle A of order m (the uniform distribution of
+ * rocblas_amd_{s,d}synth_tri under factor_seed, in the request's uplo; kept
+ * resident per factor_id and evicted least recently used when the arena is
+ * full) and n synthetic right-hand-side vectors (rocblas_amd_{s,d}synth_rhs
+ * under rhs_seed; B is m x n for side left, n x m for side right).
+ * Requests are admitted FIFO once they have arrived and their memory fits,
+ * up to max_inflight at once; those with m <= small_k are solved together in
+ * one variable-size batched launch per precision (every case and alpha in
+ * the same launch) on the first stream, larger ones by rocblas_{s,d}trsm
+ * over the other streams in turn, so small requests never wait behind large
+ * solves and new work overlaps work in flight.  With verify != 0 every
+ * request's normwise backward error
+ *   ||alpha B - op(A) X||_1 / (||op(A)||_1 ||X||_1 + |alpha| ||B||_1)
+ * is computed on the device in FP64 right after its solve, as part of the
+ * request (counted in its latency), and returned per request.  latency[i] is
+ * the host time from arrival to observed completion (-1 if rejected).  No
  * counterpart in the reference. */
 typedef struct rocblas_amd_server_* rocblas_amd_server;
 
@@ -207,10 +215,15 @@ typedef struct rocblas_amd_request_
     uint64_t factor_seed; /* seed of the factor when it is generated */
     uint64_t rhs_seed;    /* seed of B */
     double   arrival;     /* seconds after the start of the run; non-decreasing */
-    int32_t  m;           /* triangle size */
-    int32_t  n;           /* right-hand sides */
+    int32_t  m;           /* triangle order */
+    int32_t  n;           /* right-hand-side vectors */
     int32_t  precision;   /* 0: FP64, 1: FP32 */
     int32_t  keep;        /* keep X on the device for rocblas_amd_server_result */
+    int32_t  side;        /* rocblas_side (0: left) */
+    int32_t  uplo;        /* rocblas_fill (0: lower); one factor id, one uplo */
+    int32_t  transA;      /* rocblas_operation (0: none) */
+    int32_t  diag;        /* rocblas_diagonal (0: non-unit) */
+    double   alpha;       /* 0 gives X = 0 without reading A */
 } rocblas_amd_request;
 
 typedef struct rocblas_amd_serve_stats_
@@ -222,27 +235,36 @@ typedef struct rocblas_amd_serve_stats_
     int64_t factor_hits;
     int64_t factor_misses;
     int64_t evictions;
+    int64_t checked;        /* requests whose backward error was computed */
+    int64_t over_tolerance; /* of those, above 1e-12 (FP64) or 1e-5 (FP32) */
     double  wall_seconds;
     double  p50_latency; /* seconds */
     double  p99_latency;
     double  max_latency;
     double  mean_latency;
     double  peak_arena_bytes;
+    double  max_backward_error_fp64;
+    double  max_backward_error_fp32;
 } rocblas_amd_serve_stats;
 
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_create(rocblas_amd_server* server,
                                                             size_t      hbm_budget_bytes,
                                                             rocblas_int streams,
                                                             rocblas_int max_inflight,
-                                                            rocblas_int small_k);
+                                                            rocblas_int small_k,
+                                                            rocblas_int verify);
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_destroy(rocblas_amd_server server);
+/* backward_error (host, count entries, may be null): per request, -1 when not
+ * computed (verify off, or rejected). */
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_run(rocblas_amd_server         server,
                                                          const rocblas_amd_request* requests,
                                                          int64_t                    count,
                                                          double*                    latency,
+                                                         double*                    backward_error,
                                                          rocblas_amd_serve_stats*   stats);
 /* Copies the kept X of request `index` of the last run to host memory X
- * (m x n, leading dimension ldx, in the request's precision). */
+ * (B's shape: m x n for side left, n x m for side right; leading dimension
+ * ldx; in the request's precision). */
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_result(rocblas_amd_server server,
                                                             int64_t            index,
                                                             void*              X,

@@ -359,7 +359,7 @@ struct DmaOperand
     }
 };
 
-template <int BK, int NS, int OPA, int OPB>
+template <int BK, int NS, int OPA, int OPB, bool TRI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     gemm_dma_kernel(const GemmArgsT<double, double, double> a)
 {
@@ -394,7 +394,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     const int     gsz    = min(a.tiles_m - fm, GROUP);
     const int     m0     = (fm + (bid % gsz)) * 128;
     const int     n0     = ((bid % (GROUP * a.tiles_n)) / gsz) * 128;
-    const int     ktiles = a.K / BK;
+    // A triangular operand (square, K x K): this tile takes only the K range
+    // in which the operand is dense, short of the diagonal block of its
+    // rows (A) or columns (B); launch_gemm adds the diagonal blocks with the
+    // masked kernel.  m0 and n0 are multiples of 128, so the range stays a
+    // whole number of K steps.
+    int kbeg = 0, kend = a.K;
+    if constexpr(TRI)
+    {
+        if(a.tri & GT_A_LOWER)
+            kend = min(kend, m0);
+        if(a.tri & GT_A_UPPER)
+            kbeg = max(kbeg, m0 + 128);
+        if(a.tri & GT_B_LOWER)
+            kbeg = max(kbeg, n0 + 128);
+        if(a.tri & GT_B_UPPER)
+            kend = min(kend, n0);
+    }
+    const int ktiles = kend > kbeg ? (kend - kbeg) / BK : 0;
 
     const typename OA::Lane la = OA::lane_state(w, lane, m0, a.M);
     const typename OB::Lane lb = OB::lane_state(w, lane, n0, a.N);
@@ -408,7 +425,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         // tiles past the last one re-read the last tile (in bounds) into a
         // stage nobody reads any more, so every step waits the same way
         auto issue = [&](int kt, int st) {
-            const int64_t k0 = int64_t(min(kt, ktiles - 1)) * BK;
+            const int64_t k0 = (TRI ? kbeg : 0) + int64_t(min(kt, ktiles - 1)) * BK;
 #pragma unroll
             for(int g = 0; g < PER; ++g)
                 dma16(OA::src(Ap, la, g, w, k0, a.lda, a.M), &As[st][OA::dst(g, w)]);
@@ -714,17 +731,72 @@ static bool gemm_dma_eligible(const GemmArgsT<float, float, float>& a, int bk)
            && al16(a.B.base + a.B.offset) && a.A.stride % 4 == 0 && a.B.stride % 4 == 0;
 }
 
-// The DMA kernel's conditions: plain FP64 operands, 16-byte aligned rows.
+// The DMA kernel's conditions: plain FP64 operands, 16-byte aligned rows; a
+// triangular operand must be square (K x K) with one instance, and the
+// kernel then leaves its diagonal blocks to the caller.
 static bool gemm_dma_eligible(const GemmArgsT<double, double, double>& a, int bk)
 {
     auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
     // an x-contiguous operand is loaded two rows of op() at a time
     const bool a_pairs = a.opA == 1 || (a.M >= 2 && a.M % 2 == 0);
     const bool b_pairs = a.opB == 0 || (a.N >= 2 && a.N % 2 == 0);
-    return !a.tri && a_pairs && b_pairs && a.K >= bk && a.K % bk == 0 && a.lda % 2 == 0
+    const bool tri_ok  = !a.tri
+                        || (a.batch == 1 && !(a.opA == 1 && a.opB == 1)
+                            && (!(a.tri & (GT_A_LOWER | GT_A_UPPER)) || a.M == a.K)
+                            && (!(a.tri & (GT_B_LOWER | GT_B_UPPER)) || a.N == a.K));
+    return tri_ok && a_pairs && b_pairs && a.K >= bk && a.K % bk == 0 && a.lda % 2 == 0
            && a.ldb % 2 == 0 && !a.A.arr && !a.B.arr
            && al16(a.A.base + a.A.offset) && al16(a.B.base + a.B.offset) && a.A.stride % 2 == 0
            && a.B.stride % 2 == 0;
+}
+
+// The diagonal blocks of a triangular operand that the DMA kernel left out:
+// C[block] += alpha op(A)[block] op(B)[block] over the 128-wide blocks of K,
+// on the masked kernel (the full blocks as one strided batch, then the
+// partial one).  beta is already applied.
+static rocblas_status gemm_tri_diagonal(rocblas_handle h, const GemmArgsT<double, double, double>& a)
+{
+    const bool    onA  = a.tri & (GT_A_LOWER | GT_A_UPPER);
+    const int     full = a.K / 128, part = a.K % 128;
+    for(int pass = 0; pass < 2; ++pass)
+    {
+        const int nblk = pass == 0 ? full : (part ? 1 : 0);
+        const int bs   = pass == 0 ? 128 : part;
+        if(!nblk)
+            continue;
+        const int64_t k0 = pass == 0 ? 0 : int64_t(full) * 128;
+        GemmArgsT<double, double, double> d = a;
+        d.beta  = 1.0;
+        d.K     = bs;
+        d.batch = nblk;
+        d.skip_if_zero = a.skip_if_zero;
+        // the triangular operand's diagonal block k0 + 128 b (either storage
+        // order), the dense operand's matching K rows / columns, C's block
+        const int64_t tstep = 128 * (int64_t(onA ? a.lda : a.ldb) + 1);
+        if(onA)
+        {
+            d.M = bs;
+            d.A = BatchPtr<const double>{nullptr, a.A.base, tstep,
+                                         a.A.offset + k0 * (int64_t(a.lda) + 1)};
+            const int64_t bst = a.opB == 0 ? 128 : 128 * int64_t(a.ldb);
+            d.B = BatchPtr<const double>{nullptr, a.B.base, bst, a.B.offset + k0 / 128 * bst};
+            d.C = BatchPtr<double>{nullptr, a.C.base, 128, a.C.offset + k0};
+        }
+        else
+        {
+            d.N = bs;
+            d.B = BatchPtr<const double>{nullptr, a.B.base, tstep,
+                                         a.B.offset + k0 * (int64_t(a.ldb) + 1)};
+            const int64_t ast = a.opA == 0 ? 128 * int64_t(a.lda) : 128;
+            d.A = BatchPtr<const double>{nullptr, a.A.base, ast, a.A.offset + k0 / 128 * ast};
+            d.C = BatchPtr<double>{nullptr, a.C.base, 128 * int64_t(a.ldc),
+                                   a.C.offset + k0 * a.ldc};
+        }
+        const rocblas_status st = launch_gemm(h, d);
+        if(st != rocblas_status_success)
+            return st;
+    }
+    return rocblas_status_success;
 }
 
 // MFMA type of an operand pair: FP64 only when both operands are FP64.
@@ -798,18 +870,22 @@ static rocblas_status launch_gemm_cfg(rocblas_handle h, GemmArgsT<TA, TB, TC> a)
     return launch_status();
 }
 
-template <int BK, int NS>
+template <int BK, int NS, bool TRI>
 static void launch_gemm_dma_ops(rocblas_handle h, const GemmArgsT<double, double, double>& d,
                                 dim3 grid)
 {
     if(d.opA == 0 && d.opB == 0)
-        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 0, 0>), grid, dim3(256), 0, h->stream, d);
+        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 0, 0, TRI>), grid, dim3(256), 0, h->stream,
+                           d);
     else if(d.opA == 1 && d.opB == 0)
-        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 1, 0>), grid, dim3(256), 0, h->stream, d);
+        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 1, 0, TRI>), grid, dim3(256), 0, h->stream,
+                           d);
     else if(d.opA == 0 && d.opB == 1)
-        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 0, 1>), grid, dim3(256), 0, h->stream, d);
-    else
-        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 1, 1>), grid, dim3(256), 0, h->stream, d);
+        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 0, 1, TRI>), grid, dim3(256), 0, h->stream,
+                           d);
+    else if constexpr(!TRI)
+        hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 1, 1, TRI>), grid, dim3(256), 0, h->stream,
+                           d);
 }
 
 template <int BK, int NS>
@@ -838,10 +914,15 @@ static void launch_gemm_dma(rocblas_handle h, const GemmArgsT<float, float, floa
 static void launch_gemm_dma(rocblas_handle h, const GemmArgsT<double, double, double>& d,
                             dim3 grid, int bk)
 {
-    if(bk == 16)
-        launch_gemm_dma_ops<16, 2>(h, d, grid);
+    // triangular operands: the 8-deep kernel only (the 16-deep one needs
+    // more registers than its two waves per SIMD allow once the per-tile K
+    // range is added, and would spill)
+    if(d.tri)
+        launch_gemm_dma_ops<8, 3, true>(h, d, grid);
+    else if(bk == 16)
+        launch_gemm_dma_ops<16, 2, false>(h, d, grid);
     else
-        launch_gemm_dma_ops<8, 3>(h, d, grid);
+        launch_gemm_dma_ops<8, 3, false>(h, d, grid);
 }
 
 // ROCBLAS_AMD_GEMM_DMA (for A/B comparisons): 0 keeps every GEMM on
@@ -900,7 +981,7 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
                      && std::is_same_v<TC, double>)
         {
             const int mode = gemm_dma_mode();
-            const int bk   = mode == 16 && a.K % 16 == 0 ? 16 : 8;
+            const int bk   = mode == 16 && a.K % 16 == 0 && !a.tri ? 16 : 8;
             if(mode && gemm_dma_eligible(a, bk))
             {
                 GemmArgsT<TA, TB, TC> d = a;
@@ -908,7 +989,10 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
                 d.tiles_n               = (a.N + 127) / 128;
                 const dim3 grid(unsigned(int64_t(d.tiles_m) * d.tiles_n), grid_batch(a.batch));
                 launch_gemm_dma(h, d, grid, bk);
-                return launch_status();
+                const rocblas_status st = launch_status();
+                if(st != rocblas_status_success || !a.tri)
+                    return st;
+                return gemm_tri_diagonal(h, a);
             }
         }
         if constexpr(std::is_same_v<mfma_t<TA, TB>, double>)

@@ -22,12 +22,22 @@
 //     Its right-hand side buffer is freed (or kept for
 //     rocblas_amd_server_result) and its factor stays resident.
 //
-// Requests are side left, lower, no transpose, non-unit, alpha = 1, with the
-// factor drawn from rocblas_amd_{s,d}synth_tri (uniform) under the factor's
-// seed and the right-hand side from rocblas_amd_{s,d}synth_rhs under the
-// request's seed, so a request's result does not depend on what else was in
-// flight: every kernel it goes through treats instances independently.
+// Each request has its own side, uplo, transA, diag and alpha; the batched
+// launches take a case and an alpha per instance.  The factor comes from
+// rocblas_amd_{s,d}synth_tri (uniform) under the factor's seed and the
+// right-hand side from rocblas_amd_{s,d}synth_rhs under the request's seed,
+// so a request's result does not depend on what else was in flight: every
+// kernel it goes through treats instances independently.
+//
+// Verification (verify != 0): the right-hand side is generated twice, into B
+// (solved in place) and B0, and after the solve of a group the batched
+// backward-error kernels (residual.hip) compute every request's
+// ||alpha B0 - op(A) X||_1 / (||op(A)||_1 ||X||_1 + |alpha| ||B0||_1) in FP64 on
+// the same stream; the values are copied to pinned host memory before the
+// group's completion event, so they are read when the request retires.
 #include "internal.hpp"
+#include "residual.hpp"
+#include "vbatched.hpp"
 
 #include "../../include/rocblas_amd_ext.h"
 
@@ -44,6 +54,7 @@
 
 namespace
 {
+using namespace rbamd;
 using clock_type = std::chrono::steady_clock;
 
 // Best-fit sub-allocator over one device allocation (offsets, 256-byte units).
@@ -132,6 +143,7 @@ struct Factor
     size_t     bytes     = 0;
     int        m         = 0;
     int        prec      = 0;
+    int        uplo      = 0;
     int        users     = 0;       // admitted, not yet retired requests (pins it)
     uint64_t   last      = 0;       // LRU stamp
     bool       generated = false;   // synthesis launched
@@ -143,17 +155,68 @@ struct Launch
 {
     hipEvent_t           done = nullptr;
     std::vector<int64_t> reqs;
-    // vbatched argument block (device arena) and its pinned host staging
-    size_t args_off   = 0;
-    size_t args_bytes = 0;
-    void*  host_args  = nullptr;
-    size_t host_bytes = 0;
+    // argument block (device arena) and its pinned host staging
+    size_t  args_off   = 0;
+    size_t  args_bytes = 0;
+    void*   host_args  = nullptr;
+    size_t  host_bytes = 0;
+    double* host_eta   = nullptr; // backward errors, copied back (verify)
 };
 
 size_t elt(int prec)
 {
     return prec == 0 ? sizeof(double) : sizeof(float);
 }
+
+size_t round256(size_t b)
+{
+    return (b + 255) / 256 * 256;
+}
+
+// rows x cols of B for a request (m: triangle order, n: vectors)
+int b_rows(const rocblas_amd_request& r)
+{
+    return r.side == rocblas_side_left ? r.m : r.n;
+}
+int b_cols(const rocblas_amd_request& r)
+{
+    return r.side == rocblas_side_left ? r.n : r.m;
+}
+size_t b_bytes(const rocblas_amd_request& r)
+{
+    return round256(size_t(r.m) * r.n * elt(r.precision));
+}
+
+// Sections of one launch's argument block, in bytes from its start; the
+// layout is the same in the pinned staging and in the device copy.
+struct ArgLayout
+{
+    size_t vm, vn, kk, ld, fl, nv, alpha, pa, pb, pb0, acc_off, items, h2d_bytes;
+    size_t acc, eta, total;
+    explicit ArgLayout(int64_t nb, int64_t nitems, int64_t acc_doubles, size_t et)
+    {
+        auto take = [&](size_t& at, size_t bytes) {
+            at = total;
+            total += (bytes + 15) / 16 * 16;
+        };
+        total = 0;
+        take(vm, 4 * nb);
+        take(vn, 4 * nb);
+        take(kk, 4 * nb);
+        take(ld, 4 * nb);
+        take(fl, 4 * nb);
+        take(nv, 4 * nb);
+        take(alpha, et * nb);
+        take(pa, sizeof(void*) * nb);
+        take(pb, sizeof(void*) * nb);
+        take(pb0, sizeof(void*) * nb);
+        take(acc_off, 8 * nb);
+        take(items, 12 * nitems);
+        h2d_bytes = total;
+        take(acc, 8 * acc_doubles);
+        take(eta, 8 * nb);
+    }
+};
 } // namespace
 
 struct rocblas_amd_server_
@@ -166,6 +229,7 @@ struct rocblas_amd_server_
     int                         next_stream  = 0;
     int                         max_inflight = 256;
     int                         small_k      = 512;
+    bool                        verify       = true;
     std::unordered_map<int64_t, Factor> factors;
     // resident factors no admitted request uses, least recently used first:
     // (last, factor id); the eviction candidates
@@ -178,6 +242,7 @@ struct rocblas_amd_server_
     std::vector<size_t>              b_off;
     std::vector<char>                b_live;
     std::vector<char>                kept;
+    double*                          eta_out = nullptr; // caller's array (may be null)
     rocblas_amd_serve_stats          stats{};
     // recycled pinned staging buffers (bytes, pointer) and events
     std::vector<std::pair<size_t, void*>> pinned_free;
@@ -231,18 +296,21 @@ bool evict_for(Server* s, size_t bytes, size_t* off)
 
 rocblas_status synth_factor(rocblas_handle h, const rocblas_amd_request& r, void* A)
 {
+    const rocblas_fill uplo = rocblas_fill(r.uplo);
     if(r.precision == 0)
-        return rocblas_amd_dsynth_tri(h, rocblas_fill_lower, rocblas_amd_synth_uniform, r.m,
-                                      r.factor_seed, 1.0, 0.0, static_cast<double*>(A), r.m);
-    return rocblas_amd_ssynth_tri(h, rocblas_fill_lower, rocblas_amd_synth_uniform, r.m,
-                                  r.factor_seed, 1.0, 0.0, static_cast<float*>(A), r.m);
+        return rocblas_amd_dsynth_tri(h, uplo, rocblas_amd_synth_uniform, r.m, r.factor_seed, 1.0,
+                                      0.0, static_cast<double*>(A), r.m);
+    return rocblas_amd_ssynth_tri(h, uplo, rocblas_amd_synth_uniform, r.m, r.factor_seed, 1.0,
+                                  0.0, static_cast<float*>(A), r.m);
 }
 
 rocblas_status synth_rhs(rocblas_handle h, const rocblas_amd_request& r, void* B)
 {
+    const int rows = b_rows(r), cols = b_cols(r);
     if(r.precision == 0)
-        return rocblas_amd_dsynth_rhs(h, r.m, r.n, r.rhs_seed, 2, static_cast<double*>(B), r.m);
-    return rocblas_amd_ssynth_rhs(h, r.m, r.n, r.rhs_seed, 2, static_cast<float*>(B), r.m);
+        return rocblas_amd_dsynth_rhs(h, rows, cols, r.rhs_seed, 2, static_cast<double*>(B),
+                                      rows);
+    return rocblas_amd_ssynth_rhs(h, rows, cols, r.rhs_seed, 2, static_cast<float*>(B), rows);
 }
 
 double seconds_since(clock_type::time_point t0)
@@ -280,6 +348,132 @@ bool event_acquire(Server* s, hipEvent_t* e)
     return hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
 }
 
+template <typename T>
+rocblas_status solve_one(rocblas_handle h, const rocblas_amd_request& r, const void* A, void* B)
+{
+    const T alpha = T(r.alpha);
+    if constexpr(sizeof(T) == sizeof(double))
+        return rocblas_dtrsm(h, rocblas_side(r.side), rocblas_fill(r.uplo),
+                             rocblas_operation(r.transA), rocblas_diagonal(r.diag), b_rows(r),
+                             b_cols(r), &alpha, static_cast<const double*>(A), r.m,
+                             static_cast<double*>(B), b_rows(r));
+    else
+        return rocblas_strsm(h, rocblas_side(r.side), rocblas_fill(r.uplo),
+                             rocblas_operation(r.transA), rocblas_diagonal(r.diag), b_rows(r),
+                             b_cols(r), &alpha, static_cast<const float*>(A), r.m,
+                             static_cast<float*>(B), b_rows(r));
+}
+
+// Solves (and, with verify, measures) one group of one precision: the
+// argument block is staged in pinned memory and copied to the arena on the
+// group's stream.
+template <typename T>
+rocblas_status solve_group(Server* s, Launch& L, int si, bool batched)
+{
+    rocblas_handle     h    = s->handles[si];
+    hipStream_t        strm = s->streams[si];
+    const auto&        grp  = L.reqs;
+    const int64_t      nb   = int64_t(grp.size());
+    int64_t            nitems = 0, acc_doubles = 0;
+    if(s->verify)
+        for(int64_t id : grp)
+        {
+            const rocblas_amd_request& r = s->reqs[id];
+            nitems += eta_items(r.m, r.n);
+            acc_doubles += eta_acc_doubles(r.m, r.n, r.side == rocblas_side_left);
+        }
+    const ArgLayout lay(nb, nitems, acc_doubles, sizeof(T));
+    L.args_bytes = lay.total;
+    if(!evict_for(s, L.args_bytes, &L.args_off))
+        return rocblas_status_memory_error;
+    if(!pinned_acquire(s, lay.total, &L.host_args, &L.host_bytes))
+        return rocblas_status_memory_error;
+    char* hb = static_cast<char*>(L.host_args);
+    char* db = static_cast<char*>(dev_ptr(s, L.args_off));
+    auto  hi = [&](size_t at) { return reinterpret_cast<int*>(hb + at); };
+    int*      vm   = hi(lay.vm);
+    int*      vn   = hi(lay.vn);
+    int*      kk   = hi(lay.kk);
+    int*      ld   = hi(lay.ld);
+    int*      fl   = hi(lay.fl);
+    int*      nv   = hi(lay.nv);
+    T*        al   = reinterpret_cast<T*>(hb + lay.alpha);
+    void**    pa   = reinterpret_cast<void**>(hb + lay.pa);
+    void**    pb   = reinterpret_cast<void**>(hb + lay.pb);
+    void**    pb0  = reinterpret_cast<void**>(hb + lay.pb0);
+    int64_t*  aoff = reinterpret_cast<int64_t*>(hb + lay.acc_off);
+    int*      it   = hi(lay.items);
+    int64_t   acc_at = 0;
+    for(int64_t i = 0; i < nb; ++i)
+    {
+        const rocblas_amd_request& r = s->reqs[grp[i]];
+        vm[i]  = b_rows(r);
+        vn[i]  = b_cols(r);
+        kk[i]  = r.m;
+        ld[i]  = b_rows(r);
+        fl[i]  = trsm_case_flags(rocblas_side(r.side), rocblas_fill(r.uplo),
+                                rocblas_operation(r.transA), rocblas_diagonal(r.diag));
+        nv[i]  = r.n;
+        al[i]  = T(r.alpha);
+        pa[i]  = dev_ptr(s, s->factors[r.factor_id].off);
+        pb[i]  = dev_ptr(s, s->b_off[grp[i]]);
+        pb0[i] = dev_ptr(s, s->b_off[grp[i]] + b_bytes(r));
+        aoff[i] = acc_at;
+        if(s->verify)
+        {
+            acc_at += eta_acc_doubles(r.m, r.n, r.side == rocblas_side_left);
+            eta_fill_items(int(i), r.m, r.n, it);
+            it += 3 * eta_items(r.m, r.n);
+        }
+    }
+    if(hipMemcpyAsync(db, hb, lay.h2d_bytes, hipMemcpyHostToDevice, strm) != hipSuccess)
+        return rocblas_status_internal_error;
+    rocblas_status st;
+    auto di = [&](size_t at) { return reinterpret_cast<const int*>(db + at); };
+    if(batched)
+    {
+        st = vbatched_cases<T>(h, di(lay.vm), di(lay.vn), di(lay.kk), di(lay.ld),
+                               reinterpret_cast<const T* const*>(db + lay.pa),
+                               reinterpret_cast<T* const*>(db + lay.pb), di(lay.fl),
+                               reinterpret_cast<const T*>(db + lay.alpha), int(nb));
+        if(st != rocblas_status_success)
+            return st;
+        s->stats.batched_launches++;
+    }
+    else
+        for(int64_t i = 0; i < nb; ++i)
+            if((st = solve_one<T>(h, s->reqs[grp[i]], pa[i], pb[i])) != rocblas_status_success)
+                return st;
+    if(s->verify)
+    {
+        EtaArgs<T> e{};
+        e.K         = di(lay.kk);
+        e.nvec      = di(lay.nv);
+        e.flags     = di(lay.fl);
+        e.lda       = di(lay.kk);
+        e.ldx       = di(lay.ld);
+        e.ldb       = di(lay.ld);
+        e.A         = reinterpret_cast<const T* const*>(db + lay.pa);
+        e.X         = reinterpret_cast<const T* const*>(db + lay.pb);
+        e.B0        = reinterpret_cast<const T* const*>(db + lay.pb0);
+        e.alpha     = reinterpret_cast<const T*>(db + lay.alpha);
+        e.acc_off   = reinterpret_cast<const int64_t*>(db + lay.acc_off);
+        e.acc       = reinterpret_cast<double*>(db + lay.acc);
+        e.acc_total = acc_doubles;
+        e.items     = di(lay.items);
+        e.nitems    = int(nitems);
+        e.batch     = int(nb);
+        e.eta       = reinterpret_cast<double*>(db + lay.eta);
+        if((st = eta_launch<T>(strm, e)) != rocblas_status_success)
+            return st;
+        L.host_eta = reinterpret_cast<double*>(hb + lay.eta);
+        if(hipMemcpyAsync(L.host_eta, db + lay.eta, 8 * nb, hipMemcpyDeviceToHost, strm)
+           != hipSuccess)
+            return rocblas_status_internal_error;
+    }
+    return rocblas_status_success;
+}
+
 // Launch one group of admitted requests of one precision.  Batched groups of
 // small solves go to stream 0, large solves rotate over the other streams, so
 // a small request never queues behind a large solve on its stream.
@@ -297,8 +491,9 @@ rocblas_status launch_group(Server* s, const std::vector<int64_t>& group, bool b
     Launch         L;
     L.reqs = group;
     rocblas_status st;
-    // generate inputs (factor on a miss, right-hand side always) on this
-    // stream; a factor generated on another stream is waited for
+    // generate inputs (factor on a miss, right-hand side always, twice with
+    // verification) on this stream; a factor generated on another stream is
+    // waited for
     for(int64_t id : group)
     {
         const rocblas_amd_request& r = s->reqs[id];
@@ -320,91 +515,14 @@ rocblas_status launch_group(Server* s, const std::vector<int64_t>& group, bool b
             return rocblas_status_internal_error;
         if((st = synth_rhs(h, r, dev_ptr(s, s->b_off[id]))) != rocblas_status_success)
             return st;
-    }
-    if(batched)
-    {
-        const int64_t nb   = int64_t(group.size());
-        const size_t  ints = 4 * sizeof(int) * nb, ptrs = 2 * sizeof(void*) * nb;
-        L.args_bytes       = ints + ptrs + 256;
-        if(!evict_for(s, L.args_bytes, &L.args_off))
-            return rocblas_status_memory_error;
-        if(!pinned_acquire(s, L.args_bytes, &L.host_args, &L.host_bytes))
-            return rocblas_status_memory_error;
-        char*  hb  = static_cast<char*>(L.host_args);
-        int*   mi  = reinterpret_cast<int*>(hb);
-        int*   ni  = mi + nb;
-        int*   la  = ni + nb;
-        int*   lb  = la + nb;
-        size_t pof = (ints + 15) / 16 * 16;
-        void** pa  = reinterpret_cast<void**>(hb + pof);
-        void** pb  = pa + nb;
-        for(int64_t i = 0; i < nb; ++i)
-        {
-            const rocblas_amd_request& r = s->reqs[group[i]];
-            mi[i] = r.m;
-            ni[i] = r.n;
-            la[i] = r.m;
-            lb[i] = r.m;
-            pa[i] = dev_ptr(s, s->factors[r.factor_id].off);
-            pb[i] = dev_ptr(s, s->b_off[group[i]]);
-        }
-        char* db = static_cast<char*>(dev_ptr(s, L.args_off));
-        if(hipMemcpyAsync(db, hb, pof + ptrs, hipMemcpyHostToDevice, s->streams[si])
-           != hipSuccess)
-            return rocblas_status_internal_error;
-        const int* dm  = reinterpret_cast<const int*>(db);
-        const int* dn  = dm + nb;
-        const int* dla = dn + nb;
-        const int* dlb = dla + nb;
-        if(prec == 0)
-        {
-            const double one = 1.0;
-            st = rocblas_amd_dtrsm_vbatched(
-                h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                rocblas_diagonal_non_unit, dm, dn, &one,
-                reinterpret_cast<const double* const*>(db + pof), dla,
-                reinterpret_cast<double* const*>(db + pof + sizeof(void*) * nb), dlb, int(nb));
-        }
-        else
-        {
-            const float one = 1.0f;
-            st = rocblas_amd_strsm_vbatched(
-                h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                rocblas_diagonal_non_unit, dm, dn, &one,
-                reinterpret_cast<const float* const*>(db + pof), dla,
-                reinterpret_cast<float* const*>(db + pof + sizeof(void*) * nb), dlb, int(nb));
-        }
-        if(st != rocblas_status_success)
+        if(s->verify
+           && (st = synth_rhs(h, r, dev_ptr(s, s->b_off[id] + b_bytes(r))))
+                  != rocblas_status_success)
             return st;
-        s->stats.batched_launches++;
     }
-    else
-    {
-        for(int64_t id : group)
-        {
-            const rocblas_amd_request& r = s->reqs[id];
-            void* A = dev_ptr(s, s->factors[r.factor_id].off);
-            void* B = dev_ptr(s, s->b_off[id]);
-            if(prec == 0)
-            {
-                const double one = 1.0;
-                st = rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower,
-                                   rocblas_operation_none, rocblas_diagonal_non_unit, r.m, r.n,
-                                   &one, static_cast<const double*>(A), r.m,
-                                   static_cast<double*>(B), r.m);
-            }
-            else
-            {
-                const float one = 1.0f;
-                st = rocblas_strsm(h, rocblas_side_left, rocblas_fill_lower,
-                                   rocblas_operation_none, rocblas_diagonal_non_unit, r.m, r.n,
-                                   &one, static_cast<const float*>(A), r.m,
-                                   static_cast<float*>(B), r.m);
-            }
-            if(st != rocblas_status_success)
-                return st;
-        }
-    }
+    st = prec == 0 ? solve_group<double>(s, L, si, batched) : solve_group<float>(s, L, si, batched);
+    if(st != rocblas_status_success)
+        return st;
     if(!event_acquire(s, &L.done) || hipEventRecord(L.done, s->streams[si]) != hipSuccess)
         return rocblas_status_internal_error;
     s->stats.launches++;
@@ -412,19 +530,37 @@ rocblas_status launch_group(Server* s, const std::vector<int64_t>& group, bool b
     return rocblas_status_success;
 }
 
+size_t b_alloc(const Server* s, const rocblas_amd_request& r)
+{
+    return (s->verify ? 2 : 1) * b_bytes(r);
+}
+
 void retire(Server* s, Launch& L, double now, double* latency)
 {
-    for(int64_t id : L.reqs)
+    for(size_t j = 0; j < L.reqs.size(); ++j)
     {
-        const rocblas_amd_request& r = s->reqs[id];
-        latency[id]                  = now - r.arrival;
+        const int64_t              id = L.reqs[j];
+        const rocblas_amd_request& r  = s->reqs[id];
+        latency[id]                   = now - r.arrival;
         unpin(s, r.factor_id, s->factors[r.factor_id], true);
         if(r.keep)
             s->kept[id] = 1;
         else
         {
-            s->arena->release(s->b_off[id], size_t(r.m) * r.n * elt(r.precision));
+            s->arena->release(s->b_off[id], b_alloc(s, r));
             s->b_live[id] = 0;
+        }
+        if(L.host_eta)
+        {
+            const double e = L.host_eta[j];
+            if(s->eta_out)
+                s->eta_out[id] = e;
+            s->stats.checked++;
+            double& worst = r.precision == 0 ? s->stats.max_backward_error_fp64
+                                             : s->stats.max_backward_error_fp32;
+            worst = std::max(worst, e);
+            if(!(e <= (r.precision == 0 ? 1e-12 : 1e-5)))
+                s->stats.over_tolerance++;
         }
         s->stats.completed++;
     }
@@ -435,29 +571,47 @@ void retire(Server* s, Launch& L, double now, double* latency)
     s->events_free.push_back(L.done);
 }
 
-rocblas_status run(Server* s, const rocblas_amd_request* in, int64_t count, double* latency)
+rocblas_status run(Server* s, const rocblas_amd_request* in, int64_t count, double* latency,
+                   double* eta_out)
 {
     // drop the previous run's kept results
     for(size_t i = 0; i < s->b_live.size(); ++i)
         if(s->b_live[i])
-            s->arena->release(s->b_off[i],
-                              size_t(s->reqs[i].m) * s->reqs[i].n * elt(s->reqs[i].precision));
+            s->arena->release(s->b_off[i], b_alloc(s, s->reqs[i]));
+    s->b_live.clear();
     s->reqv.assign(in, in + count);
+    s->nreq = 0;
+    // 0 selects the default case: left, lower, no transpose, non-unit
+    for(auto& r : s->reqv)
+    {
+        r.side   = r.side ? r.side : rocblas_side_left;
+        r.uplo   = r.uplo ? r.uplo : rocblas_fill_lower;
+        r.transA = r.transA ? r.transA : rocblas_operation_none;
+        r.diag   = r.diag ? r.diag : rocblas_diagonal_non_unit;
+    }
     const rocblas_amd_request* reqs = s->reqv.data();
     s->reqs                         = reqs;
-    s->nreq                         = count;
-    s->b_off.assign(count, 0);
-    s->b_live.assign(count, 0);
-    s->kept.assign(count, 0);
+    s->eta_out                      = eta_out;
     s->stats = rocblas_amd_serve_stats{};
     for(int64_t i = 0; i < count; ++i)
     {
         latency[i] = -1.0;
+        if(eta_out)
+            eta_out[i] = -1.0;
         const auto& r = reqs[i];
         if(r.m <= 0 || r.n <= 0 || (r.precision != 0 && r.precision != 1) || r.arrival < 0.0
-           || (i && r.arrival < reqs[i - 1].arrival))
+           || (i && r.arrival < reqs[i - 1].arrival)
+           || (r.side != rocblas_side_left && r.side != rocblas_side_right)
+           || (r.uplo != rocblas_fill_lower && r.uplo != rocblas_fill_upper)
+           || (r.transA != rocblas_operation_none && r.transA != rocblas_operation_transpose
+               && r.transA != rocblas_operation_conjugate_transpose)
+           || (r.diag != rocblas_diagonal_non_unit && r.diag != rocblas_diagonal_unit))
             return rocblas_status_invalid_value;
     }
+    s->nreq = count;
+    s->b_off.assign(count, 0);
+    s->b_live.assign(count, 0);
+    s->kept.assign(count, 0);
 
     std::deque<int64_t> pending;
     // launches in flight, per stream in launch order (a stream completes its
@@ -499,7 +653,7 @@ rocblas_status run(Server* s, const rocblas_amd_request* in, int64_t count, doub
             const rocblas_amd_request& r  = reqs[id];
             const size_t               eb = elt(r.precision);
             const size_t               fbytes = size_t(r.m) * r.m * eb;
-            const size_t               bbytes = size_t(r.m) * r.n * eb;
+            const size_t               bbytes = b_alloc(s, r);
             if(fbytes + bbytes + 512 > s->arena->size())
             {
                 // can never fit: reject
@@ -511,7 +665,9 @@ rocblas_status run(Server* s, const rocblas_amd_request* in, int64_t count, doub
             }
             auto fit = s->factors.find(r.factor_id);
             bool hit = fit != s->factors.end();
-            if(hit && (fit->second.m != r.m || fit->second.prec != r.precision))
+            if(hit
+               && (fit->second.m != r.m || fit->second.prec != r.precision
+                   || fit->second.uplo != r.uplo))
                 return rocblas_status_invalid_value; // one factor id, one shape
             if(hit)
                 pin(s, r.factor_id, fit->second); // pinned from admission to retirement
@@ -552,6 +708,7 @@ rocblas_status run(Server* s, const rocblas_amd_request* in, int64_t count, doub
                 f.bytes = fbytes;
                 f.m     = r.m;
                 f.prec  = r.precision;
+                f.uplo  = r.uplo;
                 f.users = 1;
                 f.last  = ++s->stamp;
                 s->factors[r.factor_id] = f;
@@ -616,7 +773,8 @@ rocblas_status rocblas_amd_server_create(rocblas_amd_server* server,
                                          size_t              hbm_budget_bytes,
                                          rocblas_int         streams,
                                          rocblas_int         max_inflight,
-                                         rocblas_int         small_k)
+                                         rocblas_int         small_k,
+                                         rocblas_int         verify)
 try
 {
     if(!server)
@@ -628,6 +786,7 @@ try
     Server* s = new Server;
     s->max_inflight = max_inflight;
     s->small_k      = small_k;
+    s->verify       = verify != 0;
     if(hipGetDevice(&s->device) != hipSuccess
        || hipMalloc(reinterpret_cast<void**>(&s->base), hbm_budget_bytes) != hipSuccess)
     {
@@ -686,6 +845,7 @@ rocblas_status rocblas_amd_server_run(rocblas_amd_server         server,
                                       const rocblas_amd_request* requests,
                                       int64_t                    count,
                                       double*                    latency,
+                                      double*                    backward_error,
                                       rocblas_amd_serve_stats*   stats)
 try
 {
@@ -696,7 +856,7 @@ try
     if(count && (!requests || !latency))
         return rocblas_status_invalid_pointer;
     rbamd::DeviceGuard dg(server->device);
-    rocblas_status     st = run(server, requests, count, latency);
+    rocblas_status     st = run(server, requests, count, latency, backward_error);
     for(hipStream_t s : server->streams)
         (void)hipStreamSynchronize(s);
     if(stats)
@@ -720,11 +880,12 @@ rocblas_status rocblas_amd_server_result(rocblas_amd_server server, int64_t inde
     if(index < 0 || index >= server->nreq || !server->kept[index] || !server->b_live[index])
         return rocblas_status_invalid_value;
     const rocblas_amd_request& r = server->reqs[index];
-    if(ldx < r.m)
+    const int                  rows = b_rows(r), cols = b_cols(r);
+    if(ldx < rows)
         return rocblas_status_invalid_size;
     const size_t e = elt(r.precision);
-    if(hipMemcpy2D(X, size_t(ldx) * e, server->base + server->b_off[index], size_t(r.m) * e,
-                   size_t(r.m) * e, r.n, hipMemcpyDeviceToHost)
+    if(hipMemcpy2D(X, size_t(ldx) * e, server->base + server->b_off[index], size_t(rows) * e,
+                   size_t(rows) * e, cols, hipMemcpyDeviceToHost)
        != hipSuccess)
         return rocblas_status_internal_error;
     return rocblas_status_success;

@@ -30,6 +30,7 @@
 // rocblas_amd/autotune_vbatched.py).
 // Results do not depend on the order in which workers take items.
 #include "panel_trsm.hpp"
+#include "vbatched.hpp"
 #include "vbatched_tuning.hpp"
 
 #include "../../include/rocblas_amd_ext.h"
@@ -52,6 +53,10 @@ struct VbArgs
     int                flags; // SF_TA / SF_UPPER / SF_REV / SF_UNIT / SF_COLS
     T                  alpha_host;
     const T*           alpha_dev;
+    // per-instance case flags and alpha (device arrays; null: flags / alpha
+    // above for every instance)
+    const int*         flags_arr;
+    const T*           alpha_arr;
     int*               items;   // workspace: panel work items (inst * 8 + group, groups * batch)
     int*               order;   // workspace: k > 512 instances (batch ints)
     int*               counter; // workspace: [0] panel items taken, [1] large taken,
@@ -80,11 +85,23 @@ __device__ __forceinline__ int vb_cost_bucket(long long k, long long nv)
     return VB_BUCKETS - 1 - min(lg, VB_BUCKETS - 1); // descending cost
 }
 
+template <typename T>
+__device__ __forceinline__ int vb_flags(const VbArgs<T>& a, int i)
+{
+    return a.flags_arr ? a.flags_arr[i] : a.flags;
+}
+
+template <typename T>
+__device__ __forceinline__ T vb_alpha(const VbArgs<T>& a, int i)
+{
+    return a.alpha_arr ? a.alpha_arr[i] : a.alpha_dev ? *a.alpha_dev : a.alpha_host;
+}
+
 // instance i: triangle size k, vectors nv, and whether it is solvable
 template <typename T>
 __device__ __forceinline__ bool vb_shape(const VbArgs<T>& a, int i, int& k, int& nv)
 {
-    const bool cols = a.flags & SF_COLS;
+    const bool cols = vb_flags(a, i) & SF_COLS;
     const int  m = a.m[i], n = a.n[i];
     k  = cols ? m : n;
     nv = cols ? n : m;
@@ -157,7 +174,6 @@ __global__ void __launch_bounds__(panel::PNT) vb_panel_kernel(const VbArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char vb_smem[];
     __shared__ int item_s;
-    const T        alpha = a.alpha_dev ? *a.alpha_dev : a.alpha_host;
     const int      total = a.counter[2];
     for(;;)
     {
@@ -176,12 +192,14 @@ __global__ void __launch_bounds__(panel::PNT) vb_panel_kernel(const VbArgs<T> a)
         const int panels = (nv + panel::PNC - 1) / panel::PNC;
         const T*  Ap     = a.A[inst];
         T*        Bp     = a.B[inst];
+        const T   alpha  = vb_alpha(a, inst);
+        const int flags  = vb_flags(a, inst);
         for(int p = g; p < panels; p += G)
         {
             if(alpha == T(0))
-                panel::panel_zero<T>(Bp, k, nv, a.ldb[inst], a.flags, p * panel::PNC);
+                panel::panel_zero<T>(Bp, k, nv, a.ldb[inst], flags, p * panel::PNC);
             else
-                panel::panel_solve<T>(Ap, Bp, k, nv, a.lda[inst], a.ldb[inst], a.flags, alpha,
+                panel::panel_solve<T>(Ap, Bp, k, nv, a.lda[inst], a.ldb[inst], flags, alpha,
                                       p * panel::PNC, vb_smem);
         }
     }
@@ -189,15 +207,15 @@ __global__ void __launch_bounds__(panel::PNT) vb_panel_kernel(const VbArgs<T> a)
 
 // One instance, one chunk of up to 64 vectors, block size NB.
 template <typename T, int NB>
-__device__ void vb_solve_chunk(const VbArgs<T>& a, T* Ls, int K, int nvec, int c0, const T* Ap,
-                               int lda, T* Bp, int ldb, T alpha)
+__device__ void vb_solve_chunk(int flags, T* Ls, int K, int nvec, int c0, const T* Ap, int lda,
+                               T* Bp, int ldb, T alpha)
 {
     constexpr int LDL  = NB + 1; // LDS row stride of a staged block
     const int     lane = threadIdx.x;
-    const bool    tA   = a.flags & SF_TA;
-    const bool    rev  = a.flags & SF_REV;
-    const bool    unit = a.flags & SF_UNIT;
-    const bool    cols = a.flags & SF_COLS;
+    const bool    tA   = flags & SF_TA;
+    const bool    rev  = flags & SF_REV;
+    const bool    unit = flags & SF_UNIT;
+    const bool    cols = flags & SF_COLS;
     const int     vec  = c0 + lane;
     const bool    vok  = vec < nvec;
     const int     nblk = (K + NB - 1) / NB;
@@ -307,8 +325,6 @@ __global__ void __launch_bounds__(WAVE) vb_solve_kernel(const VbArgs<T> a)
 {
     __shared__ T Ls[32 * 33];
     __shared__ int item_s;
-    const T    alpha = a.alpha_dev ? *a.alpha_dev : a.alpha_host;
-    const bool cols  = a.flags & SF_COLS;
 
     const int  total = a.counter[3];
     for(;;)
@@ -320,7 +336,10 @@ __global__ void __launch_bounds__(WAVE) vb_solve_kernel(const VbArgs<T> a)
         __syncthreads();
         if(item >= total)
             break;
-        const int inst = a.order[item];
+        const int inst  = a.order[item];
+        const int flags = vb_flags(a, inst);
+        const T   alpha = vb_alpha(a, inst);
+        const bool cols = flags & SF_COLS;
         const int m    = a.m[inst];
         const int n    = a.n[inst];
         const int K    = cols ? m : n;
@@ -346,9 +365,9 @@ __global__ void __launch_bounds__(WAVE) vb_solve_kernel(const VbArgs<T> a)
                 continue;
             }
             if(K <= a.nb16_max_k)
-                vb_solve_chunk<T, 16>(a, Ls, K, nvec, c0, Ap, lda, Bp, ldb, alpha);
+                vb_solve_chunk<T, 16>(flags, Ls, K, nvec, c0, Ap, lda, Bp, ldb, alpha);
             else
-                vb_solve_chunk<T, 32>(a, Ls, K, nvec, c0, Ap, lda, Bp, ldb, alpha);
+                vb_solve_chunk<T, 32>(flags, Ls, K, nvec, c0, Ap, lda, Bp, ldb, alpha);
         }
     }
 }
@@ -420,15 +439,11 @@ rocblas_status vb_entry(rocblas_handle     handle,
     if(!m || !n || !lda || !ldb || !alpha || !A || !B)
         return rocblas_status_invalid_pointer;
 
+    DeviceGuard    dg(handle->device);
     WorkspaceLease lease(handle, ws);
-    if(!lease.ptr)
+    void*          wsp = lease.ptr;
+    if(!wsp)
         return rocblas_status_memory_error;
-
-    const bool sideL  = side == rocblas_side_left;
-    const bool tA     = sideL ? (transA != rocblas_operation_none)
-                              : (transA == rocblas_operation_none);
-    const bool upperA = uplo == rocblas_fill_upper;
-    const bool lowerM = (!upperA) != tA;
 
     VbArgs<T> a;
     a.m          = m;
@@ -438,14 +453,74 @@ rocblas_status vb_entry(rocblas_handle     handle,
     a.A          = A;
     a.B          = B;
     a.batch      = batch_count;
-    a.flags      = (tA ? SF_TA : 0) | (upperA ? SF_UPPER : 0) | (lowerM ? 0 : SF_REV)
-              | (diag == rocblas_diagonal_unit ? SF_UNIT : 0) | (sideL ? SF_COLS : 0);
+    a.flags      = trsm_case_flags(side, uplo, transA, diag);
     a.alpha_host = T(1);
     a.alpha_dev  = nullptr;
+    a.flags_arr  = nullptr;
+    a.alpha_arr  = nullptr;
     if(handle->pointer_mode == rocblas_pointer_mode_host)
         a.alpha_host = *alpha;
     else
         a.alpha_dev = alpha;
+    a.nb16_max_k = g_vb_tuning.nb16_max_k;
+    a.groups     = g_vb_tuning.groups;
+    // counters, then the large-instance list, then the panel work items
+    const size_t order_ints = size_t(batch_count);
+    a.counter    = static_cast<int*>(wsp);
+    a.order      = reinterpret_cast<int*>(static_cast<char*>(wsp) + 64);
+    a.items      = reinterpret_cast<int*>(static_cast<char*>(wsp) + 64
+                                          + roundup_ws(order_ints * sizeof(int)));
+    return vbatched_launch<T>(handle, a);
+}
+} // namespace
+
+int trsm_case_flags(rocblas_side side, rocblas_fill uplo, rocblas_operation trans,
+                    rocblas_diagonal diag)
+{
+    const bool sideL  = side == rocblas_side_left;
+    const bool tA     = sideL ? (trans != rocblas_operation_none)
+                              : (trans == rocblas_operation_none);
+    const bool upperA = uplo == rocblas_fill_upper;
+    const bool lowerM = (!upperA) != tA;
+    return (tA ? SF_TA : 0) | (upperA ? SF_UPPER : 0) | (lowerM ? 0 : SF_REV)
+           | (diag == rocblas_diagonal_unit ? SF_UNIT : 0) | (sideL ? SF_COLS : 0);
+}
+
+template <typename T>
+rocblas_status vbatched_cases(rocblas_handle     handle,
+                              const rocblas_int* m,
+                              const rocblas_int* n,
+                              const rocblas_int* lda,
+                              const rocblas_int* ldb,
+                              const T* const*    A,
+                              T* const*          B,
+                              const int*         flags,
+                              const T*           alpha,
+                              rocblas_int        batch_count)
+{
+    if(batch_count < 0 || batch_count >= (1 << 28))
+        return rocblas_status_invalid_size;
+    if(!batch_count)
+        return rocblas_status_success;
+    const size_t ws = 64 + roundup_ws(size_t(batch_count) * sizeof(int))
+                      + roundup_ws(size_t(batch_count) * 8 * sizeof(int));
+    DeviceGuard    dg(handle->device);
+    WorkspaceLease lease(handle, ws);
+    if(!lease.ptr)
+        return rocblas_status_memory_error;
+    VbArgs<T> a{};
+    a.m          = m;
+    a.n          = n;
+    a.lda        = lda;
+    a.ldb        = ldb;
+    a.A          = A;
+    a.B          = B;
+    a.batch      = batch_count;
+    a.flags      = 0;
+    a.alpha_host = T(1);
+    a.alpha_dev  = nullptr;
+    a.flags_arr  = flags;
+    a.alpha_arr  = alpha;
     a.nb16_max_k = g_vb_tuning.nb16_max_k;
     a.groups     = g_vb_tuning.groups;
     a.counter    = static_cast<int*>(lease.ptr);
@@ -454,7 +529,17 @@ rocblas_status vb_entry(rocblas_handle     handle,
                                           + roundup_ws(size_t(batch_count) * sizeof(int)));
     return vbatched_launch<T>(handle, a);
 }
-} // namespace
+
+template rocblas_status vbatched_cases<float>(rocblas_handle, const rocblas_int*,
+                                              const rocblas_int*, const rocblas_int*,
+                                              const rocblas_int*, const float* const*,
+                                              float* const*, const int*, const float*,
+                                              rocblas_int);
+template rocblas_status vbatched_cases<double>(rocblas_handle, const rocblas_int*,
+                                               const rocblas_int*, const rocblas_int*,
+                                               const rocblas_int*, const double* const*,
+                                               double* const*, const int*, const double*,
+                                               rocblas_int);
 } // namespace rbamd
 
 using namespace rbamd;

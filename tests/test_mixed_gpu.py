@@ -138,3 +138,31 @@ def test_argument_checks_and_alpha_zero(h):
     assert st == S.success
     torch.cuda.synchronize()
     assert torch.count_nonzero(dB).item() == 0
+
+
+@pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_reported_eta_through_the_dma_residual(h, side, uplo, trans, diag):
+    """At 2056 x 2048 the FP64 residual alpha B - op(A) X runs on the DMA GEMM
+    (the triangle's dense part) plus the masked kernel (its 128-wide diagonal
+    blocks, the last one 8 wide).  With no refinement the reported eta is the
+    first low-precision solve's, far above rounding level, so it must equal a
+    plain PyTorch float64 residual's to many digits: a missing or doubled
+    block, or a wrong mask, would change it at O(1)."""
+    from tests.gpu_util import eta_torch, logical
+    m, n = (2056, 2048) if side == "L" else (2048, 2056)
+    k = m if side == "L" else n
+    say("mixed dma residual", side, uplo, trans, diag)
+    g = torch.Generator(device="cuda").manual_seed(17 + ALL_CASES.index((side, uplo, trans)))
+    dA = torch.randn(k * k, dtype=torch.float64, device="cuda", generator=g) / k ** 0.5
+    dA.view(k, k).diagonal().uniform_(1.0, 2.0, generator=g)
+    dB0 = torch.rand(m * n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    dB = dB0.clone()
+    st, info = rb.trsm_mixed(h, side, uplo, trans, diag, m, n, 1.5, dA.data_ptr(), k,
+                             dB.data_ptr(), m, block=256, max_refine=0, tol=1e-30)
+    assert st == S.success and info.iterations == 0
+    torch.cuda.synchronize()
+    eta = eta_torch(side, uplo, trans, diag, 1.5, logical(dA, k, k, k), logical(dB0, m, m, n),
+                    logical(dB, m, m, n))
+    assert 1e-12 < eta < 1e-3, eta
+    assert abs(info.backward_error - eta) <= 1e-6 * eta, (info.backward_error, eta)
