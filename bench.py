@@ -63,27 +63,67 @@ def make_inputs(torch, synth):
     return data, Bsrc
 
 
-def cpu_baseline(oracle, data, Bsrc, budget_s=12.0):
-    """Oracle netlib loop nest, OpenMP over instances, all eight cases; repeats the
-    full workload until ~budget_s of CPU wall time (at least once)."""
+def cpu_baseline(oracle, data, Bsrc, budget_s=12.0, cblas=False):
+    """All eight cases of the workload on the host, OpenMP over instances,
+    repeated until ~budget_s of wall time (at least one full pass): the
+    oracle's netlib loop nest, or (cblas=True) the host CBLAS dtrsm loaded by
+    the oracle library (the reference's own host comparator is cblas_dtrsm)."""
     threads = oracle.num_threads()
+    solve = oracle.cblas_trsm_batched_omp if cblas else oracle.trsm_batched_omp
     reps, t_total, solves = 0, 0.0, 0
     while reps == 0 or t_total < budget_s:
         for (side, uplo, trans), (Af, _, _) in data.items():
             k = M if side == "L" else N
             Bw = Bsrc[side].copy()
             t0 = time.perf_counter()
-            oracle.trsm_batched_omp(side, uplo, trans, "N", M, N, 1.0, Af, k, k * k, Bw, M, M * N,
-                                    BATCH, threads)
+            solve(side, uplo, trans, "N", M, N, 1.0, Af, k, k * k, Bw, M, M * N, BATCH, threads)
             t_total += time.perf_counter() - t0
             solves += BATCH
         reps += 1
-        if reps >= 50:
+        if reps >= 2000:
             break
-    return {"value": solves / t_total, "unit": "solves/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{reps} pass(es) over all 8 cases x {BATCH} instances (m={M}, n={N}), "
-                      f"{t_total:.1f} s"}
+    out = {"value": solves / t_total, "unit": "solves/s", "cores": threads, "kind": "port",
+           "sample": f"{reps} pass(es) over all 8 cases x {BATCH} instances (m={M}, n={N}), "
+                     f"{t_total:.1f} s"}
+    if cblas:
+        out["kind"] = "host CBLAS " + oracle.cblas_load()
+    return out
+
+
+def vendor_strided(torch, data, reps=20):
+    """The vendor librocblas (shipped with torch) on the same eight launches,
+    through its C API: solves/s over reps steps."""
+    import ctypes
+    import bench_legs as bl
+    vendor = bl.vendor_lib(torch)
+    if vendor is None:
+        return None
+    V, vh = vendor
+    vp = ctypes.c_void_p
+    V.rocblas_dtrsm_strided_batched.argtypes = [vp] + [ctypes.c_int] * 6 + [
+        vp, vp, ctypes.c_int, ctypes.c_int64, vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int]
+    enums = {"L": 141, "R": 142}, {"U": 121, "L": 122}, {"N": 111, "T": 112}
+    one = ctypes.c_double(1.0)
+
+    def vstep():
+        for (side, uplo, trans), (_, dA, dB) in data.items():
+            k = M if side == "L" else N
+            st = V.rocblas_dtrsm_strided_batched(
+                vh, enums[0][side], enums[1][uplo], enums[2][trans], 131, M, N,
+                ctypes.addressof(one), vp(dA.data_ptr()), k, k * k, vp(dB.data_ptr()), M, M * N,
+                BATCH)
+            assert st == 0, st
+    for _ in range(3):
+        vstep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        vstep()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    V.rocblas_destroy_handle(vh)
+    return {"solves_per_s": len(CASES) * BATCH / t, "ms_per_step": t * 1e3,
+            "library": "librocblas.so shipped with torch, C API"}
 
 
 def check_accuracy(torch, rb, h, data, Bsrc):
@@ -106,6 +146,34 @@ def check_accuracy(torch, rb, h, data, Bsrc):
         eta = n1(R) / (n1(op) * n1(X) + n1(B0))
         worst = max(worst, float(eta.max()))
     return worst
+
+
+def headline_ratios(work):
+    """The metric's comparisons with the plain FP64 solve, lifted from the
+    workloads: the precision form over plain FP64 (TFLOP/s at 16384^2), the
+    variable-size batch's graph launch over a per-instance loop of plain
+    dtrsm (solves/s), and the latencies the metric names."""
+    r = {}
+    lg = work.get("large_16384", {})
+    plain = lg.get("plain_fp64", {}).get("tflops")
+    for form in ("mixed_fp32_refined", "mixed_bf16x3_refined"):
+        f = lg.get(form, {})
+        if plain and f.get("tflops"):
+            r[f"large_16384_{form}_over_plain_tflops"] = f["tflops"] / plain
+            r[f"large_16384_{form}_backward_error"] = f.get("backward_error")
+    vb = work.get("variable_batch_4096", {})
+    if "speedup_vs_loop" in vb:
+        r["variable_batch_graph_over_dtrsm_loop_solves"] = vb["speedup_vs_loop"]
+    if "speedup_vs_vendor_loop" in vb:
+        r["variable_batch_graph_over_vendor_dtrsm_loop_solves"] = vb["speedup_vs_vendor_loop"]
+    sv = work.get("serving_poisson_2000rps", {})
+    if "p99_ms" in sv:
+        r["serving_p50_ms"] = sv["p50_ms"]
+        r["serving_p99_ms"] = sv["p99_ms"]
+    sg = work.get("single_512x256", {}).get("ours", {})
+    if "p99_us" in sg:
+        r["single_512x256_p99_us"] = sg["p99_us"]
+    return r
 
 
 def main():
@@ -163,8 +231,30 @@ def main():
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
 
-    # per-launch durations on the launch stream (each call is one kernel launch)
+    # Per-launch durations on the launch stream (each call is one kernel
+    # launch): events bracket the four side-left launches and then the four
+    # side-right launches of a step, so one event pair is amortised over four
+    # back-to-back launches; each group's time / 4 is that kernel's average
+    # launch duration.  Single-launch event pairs give the latency
+    # percentiles.
     reps = max(10, min(args.steps, 50))
+    left_us, right_us = [], []
+    for _ in range(reps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        ev[0].record(stream)
+        for side_group, e in (("L", ev[1]), ("R", ev[2])):
+            for (side, uplo, trans), (_, dA, dB) in data.items():
+                if side != side_group:
+                    continue
+                k = M if side == "L" else N
+                rb.check(rb.trsm_strided_batched(h, side, uplo, trans, "N", M, N, 1.0,
+                                                 dA.data_ptr(), k, k * k, dB.data_ptr(), M,
+                                                 M * N, BATCH))
+            e.record(stream)
+        torch.cuda.synchronize()
+        left_us.append(ev[0].elapsed_time(ev[1]) * 1e3 / 4)
+        right_us.append(ev[1].elapsed_time(ev[2]) * 1e3 / 4)
+    left_us, right_us = np.array(left_us), np.array(right_us)
     per_case = {c: [] for c in data}
     for _ in range(reps):
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -174,8 +264,6 @@ def main():
         for idx, c in enumerate(data):
             per_case[c].append(ev[idx][0].elapsed_time(ev[idx][1]) * 1e3)  # us
     all_us = np.concatenate([np.array(v) for v in per_case.values()])
-    left_us = np.concatenate([np.array(v) for c, v in per_case.items() if c[0] == "L"])
-    right_us = np.concatenate([np.array(v) for c, v in per_case.items() if c[0] == "R"])
     left_avg_s = float(left_us.mean()) * 1e-6
     achieved = BATCH * algo_bytes("L") / left_avg_s / 1e9
 
@@ -194,8 +282,10 @@ def main():
             traffic = json.load(f)["traffic_bytes_per_launch"]
     solves = len(CASES) * BATCH
     flops = sum(BATCH * algo_flops(c[0]) for c in CASES)
+    with open(os.path.join(ROOT, "BASELINE.json")) as f:
+        metric = json.load(f)["metric"]
     out = {
-        "metric": "dtrsm solves/s (plain FP64, strided batched) at <=1e-12 backward error",
+        "metric": metric,
         "value": solves / (ms_per_step * 1e-3),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -219,12 +309,20 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "small_trsm_dpp_kernel<double,32,*,*,cols> (the 4 side-left cases)",
                      "algorithmic_bytes_per_launch": BATCH * algo_bytes("L"),
-                     "avg_launch_us": float(left_us.mean())},
+                     "avg_launch_us": float(left_us.mean()),
+                     "timing": "HIP events on the launch stream around 4 back-to-back launches"},
     }
+    if not args.no_vendor:
+        v = vendor_strided(torch, data)
+        if v is not None:
+            v["ours_over_vendor"] = out["value"] / v["solves_per_s"]
+            out["vendor_rocblas"] = v
     import oracle
     oracle.build()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(oracle, data, Bsrc)
+        if oracle.cblas_load() is not None:
+            out["cpu_baseline_cblas"] = cpu_baseline(oracle, data, Bsrc, budget_s=8.0, cblas=True)
     h.close()
     legs = {"single", "large", "vbatch", "serve"} if args.legs == "all" else \
         set() if args.legs == "none" else set(args.legs.split(","))
@@ -250,6 +348,7 @@ def main():
                 work[name] = {"error": f"{type(e).__name__}: {e}"}
             work[name]["leg_seconds"] = time.perf_counter() - t0
         out["workloads"] = work
+        out["vs_plain_fp64"] = headline_ratios(work)
     if eta > 1e-12:
         print(f"backward error {eta:.3e} above 1e-12", file=sys.stderr)
     print(json.dumps(out))

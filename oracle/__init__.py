@@ -76,6 +76,13 @@ def lib():
         L.oracle_trtri_d.restype = None
         L.oracle_trtri_s.argtypes = [i, i, i, fp, i]
         L.oracle_trtri_s.restype = None
+        L.oracle_cblas_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
+        L.oracle_cblas_load.restype = i
+        L.oracle_cblas_dtrsm.argtypes = [i, i, i, i, i, i, ctypes.c_double, dp, i, dp, i, i]
+        L.oracle_cblas_dtrsm.restype = i
+        L.oracle_cblas_dtrsm_batched_omp.argtypes = [i, i, i, i, i, i, ctypes.c_double, dp, i,
+                                                     ll, dp, i, ll, i, i]
+        L.oracle_cblas_dtrsm_batched_omp.restype = i
         _lib = L
     return _lib
 
@@ -154,6 +161,60 @@ def trsm_omp(side, uplo, trans, diag, m, n, alpha, A, B, threads=0):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+# --------------------------------------------------- host CBLAS (timing) --
+_cblas_name = None
+
+
+def cblas_load():
+    """Load a host CBLAS for the second CPU baseline: the OpenBLAS that ships
+    with scipy (``scipy_cblas_dtrsm``), else a system ``libcblas``/``libopenblas``.
+    Returns a description of what was loaded, or None."""
+    global _cblas_name
+    if _cblas_name is not None:
+        return _cblas_name
+    import glob
+    cands = []
+    try:
+        import scipy
+        libs = os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs")
+        for p in sorted(glob.glob(os.path.join(libs, "libscipy_openblas*.so"))):
+            cands.append((p, "scipy_cblas_dtrsm", "scipy_openblas_set_num_threads"))
+    except ImportError:
+        pass
+    for p in ("libopenblas.so.0", "libcblas.so.3", "libblas.so.3"):
+        cands.append((p, "cblas_dtrsm", "openblas_set_num_threads"))
+    for path, sym, thr in cands:
+        if lib().oracle_cblas_load(path.encode(), sym.encode(), thr.encode()) == 0:
+            _cblas_name = f"{os.path.basename(path)}:{sym}"
+            return _cblas_name
+    return None
+
+
+def cblas_trsm(side, uplo, trans, diag, m, n, alpha, A, B, threads=0):
+    """In-place solve through the loaded host CBLAS (float64, column-major)."""
+    _check_colmajor(A, "A")
+    _check_colmajor(B, "B")
+    if cblas_load() is None:
+        raise RuntimeError("no host CBLAS found")
+    s, u, t, d = _enums(side, uplo, trans, diag)
+    if lib().oracle_cblas_dtrsm(s, u, t, d, m, n, float(alpha), _ptr(A), A.shape[0], _ptr(B),
+                                B.shape[0], threads):
+        raise RuntimeError("host CBLAS call failed")
+    return B
+
+
+def cblas_trsm_batched_omp(side, uplo, trans, diag, m, n, alpha, A, lda, stride_a, B, ldb,
+                           stride_b, batch, threads=0):
+    """Strided batch through the host CBLAS, OpenMP over instances."""
+    if cblas_load() is None:
+        raise RuntimeError("no host CBLAS found")
+    s, u, t, d = _enums(side, uplo, trans, diag)
+    if lib().oracle_cblas_dtrsm_batched_omp(s, u, t, d, m, n, float(alpha), _ptr(A), lda,
+                                            stride_a, _ptr(B), ldb, stride_b, batch, threads):
+        raise RuntimeError("host CBLAS call failed")
+    return B
 
 
 def backward_error(side, uplo, trans, diag, m, n, alpha, A, B0, X):

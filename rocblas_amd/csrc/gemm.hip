@@ -750,10 +750,15 @@ static bool gemm_dma_eligible(const GemmArgsT<double, double, double>& a, int bk
            && a.B.stride % 2 == 0;
 }
 
+template <typename TA, typename TB, typename TC>
+static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a,
+                                      bool allow_dma);
+
 // The diagonal blocks of a triangular operand that the DMA kernel left out:
 // C[block] += alpha op(A)[block] op(B)[block] over the 128-wide blocks of K,
 // on the masked kernel (the full blocks as one strided batch, then the
-// partial one).  beta is already applied.
+// partial one).  beta is already applied.  The DMA kernel is never chosen
+// here: a single diagonal block would be eligible again and lead back here.
 static rocblas_status gemm_tri_diagonal(rocblas_handle h, const GemmArgsT<double, double, double>& a)
 {
     const bool    onA  = a.tri & (GT_A_LOWER | GT_A_UPPER);
@@ -792,7 +797,7 @@ static rocblas_status gemm_tri_diagonal(rocblas_handle h, const GemmArgsT<double
             d.C = BatchPtr<double>{nullptr, a.C.base, 128 * int64_t(a.ldc),
                                    a.C.offset + k0 * a.ldc};
         }
-        const rocblas_status st = launch_gemm(h, d);
+        const rocblas_status st = launch_gemm_sel(h, d, false);
         if(st != rocblas_status_success)
             return st;
     }
@@ -944,7 +949,8 @@ static int gemm_dma_mode()
 
 // Pick the largest tile that still gives every CU work.
 template <typename TA, typename TB, typename TC>
-rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
+static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a,
+                                      bool allow_dma)
 {
     if(a.M <= 0 || a.N <= 0)
         return rocblas_status_success;
@@ -967,7 +973,7 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
         {
             const int mode = gemm_dma_mode();
             const int bk   = mode != 8 && a.K % 32 == 0 ? 32 : 16;
-            if(mode && gemm_dma_eligible(a, bk))
+            if(allow_dma && mode && gemm_dma_eligible(a, bk))
             {
                 GemmArgsT<TA, TB, TC> d = a;
                 d.tiles_m               = (a.M + 127) / 128;
@@ -982,7 +988,7 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
         {
             const int mode = gemm_dma_mode();
             const int bk   = mode == 16 && a.K % 16 == 0 && !a.tri ? 16 : 8;
-            if(mode && gemm_dma_eligible(a, bk))
+            if(allow_dma && mode && gemm_dma_eligible(a, bk))
             {
                 GemmArgsT<TA, TB, TC> d = a;
                 d.tiles_m               = (a.M + 127) / 128;
@@ -1003,6 +1009,12 @@ rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
     if(tiles(64, 64) >= cus)
         return launch_gemm_cfg<TA, TB, TC, 64, 64, 16, 2, 2>(h, a);
     return launch_gemm_cfg<TA, TB, TC, 32, 32, 16, 2, 2>(h, a);
+}
+
+template <typename TA, typename TB, typename TC>
+rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
+{
+    return launch_gemm_sel(h, a, true);
 }
 
 template rocblas_status launch_gemm(rocblas_handle, const GemmArgsT<float, float, float>&);

@@ -437,3 +437,34 @@ def test_backward_error_float32_closed_forms():
                                 np.asfortranarray(Xr.astype(float)))
     assert oracle.backward_error("R", "U", "T", "N", n, m, 1.5, A, B0r, Xr) == pytest.approx(
         e64, rel=1e-12)
+
+
+def test_host_cblas_baseline_agrees_with_oracle():
+    """The host CBLAS that the CPU baseline also times (loaded by dlopen) solves
+    the same problems as the oracle: every case, single and strided batch; and
+    the integer problems exactly."""
+    if oracle.cblas_load() is None:
+        pytest.skip("no host CBLAS library on this machine")
+    rng = np.random.default_rng(91)
+    for (side, uplo, trans), diag in [(c, d) for c in ALL_CASES for d in "NU"]:
+        m, n = 29, 17
+        k = m if side == "L" else n
+        A = oracle.diag_dominant_triangular(k, uplo, rng)
+        B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        X1 = oracle.trsm(side, uplo, trans, diag, m, n, -0.75, A, B.copy(order="F"))
+        X2 = oracle.cblas_trsm(side, uplo, trans, diag, m, n, -0.75, A, B.copy(order="F"))
+        assert oracle.norm1_rel_diff(m, n, X2, X1) <= 40 * EPS64 * k
+    for case in ALL_CASES:
+        m, n = 24, 13
+        A, B, X = integer_kat(*case, "N", m, n, rng, poison=0.0)
+        got = oracle.cblas_trsm(*case, "N", m, n, 1.0, A, B.copy(order="F"), threads=1)
+        assert np.array_equal(got, X), case
+    batch, m, n = 50, 32, 16
+    Af = np.concatenate([oracle.diag_dominant_triangular(m, "L", rng).ravel(order="F")
+                         for _ in range(batch)])
+    Bf = rng.uniform(-1, 1, batch * m * n)
+    want = oracle.trsm_batched_omp("L", "L", "T", "N", m, n, 1.0, Af, m, m * m, Bf.copy(), m,
+                                   m * n, batch)
+    got = oracle.cblas_trsm_batched_omp("L", "L", "T", "N", m, n, 1.0, Af, m, m * m, Bf.copy(),
+                                        m, m * n, batch, 4)
+    assert np.abs(got - want).max() <= 64 * EPS64 * np.abs(want).max()
