@@ -17,7 +17,7 @@ solves, a 4096-instance variable-size batch, the Poisson serving loop) are
 measured after the main line by bench_legs.py and reported under "workloads".
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--legs all|none|single,large,vbatch,serve] [--serve-seconds S]
+                    [--legs all|none|single,large,vbatch,trtri,serve] [--serve-seconds S]
 """
 import argparse
 import json
@@ -324,7 +324,7 @@ def main():
         if oracle.cblas_load() is not None:
             out["cpu_baseline_cblas"] = cpu_baseline(oracle, data, Bsrc, budget_s=8.0, cblas=True)
     h.close()
-    legs = {"single", "large", "vbatch", "serve"} if args.legs == "all" else \
+    legs = {"single", "large", "vbatch", "trtri", "serve"} if args.legs == "all" else \
         set() if args.legs == "none" else set(args.legs.split(","))
     if legs:
         import bench_legs as bl
@@ -335,6 +335,7 @@ def main():
         runs = [("single_512x256", "single", lambda: bl.leg_single(torch, rb, synth, oracle, vendor)),
                 ("large_16384", "large", lambda: bl.leg_large(torch, rb, oracle, vendor)),
                 ("variable_batch_4096", "vbatch", lambda: bl.leg_variable_batch(torch, rb, vendor)),
+                ("trtri", "trtri", lambda: bl.leg_trtri(torch, rb, vendor)),
                 ("serving_poisson_2000rps", "serve",
                  lambda: bl.leg_serving(torch, rb, duration=args.serve_seconds))]
         for name, key, fn in runs:

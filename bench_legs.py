@@ -286,6 +286,76 @@ def leg_variable_batch(torch, rb, vendor, batch=4096, seed=0):
     return res
 
 
+def leg_trtri(torch, rb, vendor, sizes=(1024, 4096, 16384), reps=5):
+    """Public rocblas_dtrtri (lower, non-unit) at several orders, ours and the
+    vendor library's, credited n^3/3 flops (the reference's count,
+    clients/include/flops.hpp:741-744); ||inv(A) A - I||_max checked on the
+    device for each result.  Plus a strided batch of 20000 order-32 inverses."""
+    import ctypes
+    from rocblas_amd import synth
+    out = {"workload": "dtrtri lower non-unit, A = diag U[1,2], off-diagonal U[-1,1]/n",
+           "flops": "n^3/3"}
+    h = rb.Handle()
+    rb.check(h.set_stream(torch.cuda.current_stream().cuda_stream))
+    if vendor is not None:
+        V, vh = vendor
+        V.rocblas_dtrtri.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return float(np.median(ts))
+
+    for n in sizes:
+        dA = torch.empty(n * n, dtype=torch.float64, device="cuda")
+        rb.check(rb.synth_tri(h, "L", "uniform", n, 5, dA.data_ptr(), n, scale=1.0 / n),
+                 "synth_tri")
+        dX = torch.empty_like(dA)
+        t = timed(lambda: rb.check(rb.trtri(h, "L", "N", n, dA.data_ptr(), n, dX.data_ptr(), n),
+                                   "dtrtri"))
+        A = torch.tril(dA.view(n, n).T)
+        X = dX.view(n, n).T
+        err = float((X @ A - torch.eye(n, dtype=torch.float64, device="cuda")).abs().max())
+        row = {"ms": t * 1e3, "tflops": n ** 3 / 3 / t / 1e12, "max_abs_inv_times_A_minus_I": err}
+        if vendor is not None:
+            dY = torch.empty_like(dA)
+            tv = timed(lambda: V.rocblas_dtrtri(vh, 122, 131, n, dA.data_ptr(), n, dY.data_ptr(),
+                                                n))
+            row["vendor_ms"] = tv * 1e3
+            row["vendor_tflops"] = n ** 3 / 3 / tv / 1e12
+            row["ours_over_vendor"] = tv / t
+            del dY
+        out[f"n{n}"] = row
+        del dA, dX, A, X
+        torch.cuda.empty_cache()
+    n, batch = 32, 20000
+    dA = torch.empty(batch * n * n, dtype=torch.float64, device="cuda")
+    # one synthetic order-32 triangle repeated over the batch
+    rb.check(rb.synth_tri(h, "L", "uniform", n, 9, dA.data_ptr(), n, scale=1.0 / n), "synth")
+    dA.view(batch, n * n).copy_(dA[: n * n].view(1, n * n).expand(batch, n * n))
+    dX = torch.empty_like(dA)
+    t = timed(lambda: rb.check(rb.trtri_strided_batched(h, "L", "N", n, dA.data_ptr(), n, n * n,
+                                                        dX.data_ptr(), n, n * n, batch),
+                               "dtrtri_strided_batched"))
+    out["strided_batched_20000_n32"] = {
+        "us": t * 1e6, "inverses_per_s": batch / t,
+        "hbm_gbs": batch * 8 * (n * (n + 1) / 2 + n * n) / t / 1e9}
+    del dA, dX
+    h.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def _serve_once(serve, rate, duration, budget, seed=0):
     reqs = serve.poisson_trace(rate, duration, seed=seed)
     srv = serve.Server(budget=budget, streams=4, max_inflight=512)

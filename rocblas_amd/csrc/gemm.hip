@@ -837,7 +837,7 @@ static void launch_gemm_ops(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a, di
     }
     else
     {
-        if constexpr(any_ops && !TRI)
+        if constexpr(any_ops)
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, T, TRI, 1, 1, BM, BN, BK, WGM, WGN>), grid,
                                block, 0, h->stream, a);
     }
@@ -847,7 +847,8 @@ template <typename TA, typename TB, typename TC>
 static bool gemm_ops_supported(const GemmArgsT<TA, TB, TC>& a)
 {
     if constexpr(std::is_same_v<TA, TB>)
-        return !(a.tri && a.opA == 1 && a.opB == 1);
+        return true; // every transpose pair, masked or not (the inversion's doubling steps
+                     // take triangular op(A) = X^T with op(B) = A^T)
     else if constexpr(std::is_same_v<TA, float>)
         return a.opB == 0 && !a.tri;
     else

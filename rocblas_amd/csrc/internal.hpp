@@ -191,7 +191,7 @@ template <typename T>
 rocblas_status identity_launch(rocblas_handle h, int n, BatchPtr<T> X, int ldx, int batch);
 
 template <typename T>
-rocblas_status copy_launch(rocblas_handle h, int m, int n, BatchPtr<const T> S, int lds,
-                           BatchPtr<T> D, int ldd, int batch);
+rocblas_status copy_launch(rocblas_handle h, int m, int n, BatchPtr<const T> S, int64_t lds,
+                           BatchPtr<T> D, int64_t ldd, int batch);
 
 } // namespace rbamd

@@ -119,11 +119,119 @@ rocblas_status trsm_entry(rocblas_handle    handle,
     return trsm_launch<T>(handle, p);
 }
 
+// Leading dimensions beyond INT32_MAX (the reference's 64-bit substitution
+// fallback, rocblas_trsm_kernels_64.cpp:150-215): such a matrix has few
+// columns (A: k of them; B: n, or k for side right), so its used part is
+// gathered into a compact workspace copy (leading dimension k, or the rows
+// of the chunk), solved there by the 32-bit path and, for B, scattered
+// back.  One instance at a time, right-hand sides in chunks that keep the
+// copy of B under 2^28 elements.  The copies move O(size) data around the
+// usual solve.
+template <typename T>
+rocblas_status trsm64_compact(rocblas_handle    handle,
+                              rocblas_side      side,
+                              rocblas_fill      uplo,
+                              rocblas_operation transA,
+                              rocblas_diagonal  diag,
+                              int64_t           m,
+                              int64_t           n,
+                              const T*          alpha,
+                              BatchPtr<const T> A,
+                              bool              A_null,
+                              int64_t           lda,
+                              BatchPtr<T>       B,
+                              bool              B_null,
+                              int64_t           ldb,
+                              int64_t           batch_count)
+{
+    constexpr int64_t i32max = 0x7fffffff;
+    const bool        sideL  = side == rocblas_side_left;
+    const int64_t     k      = sideL ? m : n;
+    const int64_t     nv     = sideL ? n : m; // right-hand-side vectors
+    const int64_t     other  = sideL ? m : n; // length of one vector
+    const bool        cpA    = lda > i32max;
+    const bool        cpB    = ldb > i32max;
+    const int64_t     vstep  = std::max<int64_t>(
+        1, std::min<int64_t>(int64_t(1) << 30, (int64_t(1) << 28) / other));
+    const int64_t vc_max = std::min(vstep, nv);
+    const size_t  a_els  = cpA ? size_t(k) * size_t(k) : 0;
+    const size_t  b_els  = cpB ? size_t(vc_max) * size_t(other) : 0;
+    const size_t  bytes  = roundup_ws(a_els * sizeof(T)) + roundup_ws(b_els * sizeof(T));
+    if(handle->size_query)
+    {
+        if(!bytes || bytes <= handle->query_size)
+            return rocblas_status_size_unchanged;
+        handle->query_size = bytes;
+        return rocblas_status_size_increased;
+    }
+    if(B_null || !alpha
+       || (handle->pointer_mode == rocblas_pointer_mode_host && *alpha != T(0) && A_null))
+        return rocblas_status_invalid_pointer;
+    const bool skipA = A_null || (handle->pointer_mode == rocblas_pointer_mode_host
+                                  && *alpha == T(0)); // A is not referenced
+
+    DeviceGuard    dg(handle->device);
+    WorkspaceLease lease(handle, bytes);
+    if(!lease.ptr)
+        return rocblas_status_memory_error;
+    T* const Ac = static_cast<T*>(lease.ptr);
+    T* const Bc = reinterpret_cast<T*>(static_cast<char*>(lease.ptr)
+                                       + roundup_ws(a_els * sizeof(T)));
+    for(int64_t b = 0; b < batch_count; ++b)
+    {
+        BatchPtr<const T> Ab = A.arr ? BatchPtr<const T>{A.arr + b, nullptr, 0, A.offset}
+                                     : BatchPtr<const T>{nullptr, A.base, 0, A.offset + b * A.stride};
+        BatchPtr<T> Bb = B.arr ? BatchPtr<T>{B.arr + b, nullptr, 0, B.offset}
+                               : BatchPtr<T>{nullptr, B.base, 0, B.offset + b * B.stride};
+        rocblas_status st;
+        BatchPtr<const T> Au   = Ab;
+        int               ldau = cpA ? int(k) : int(lda);
+        if(cpA)
+        {
+            Au = BatchPtr<const T>{nullptr, Ac, 0, 0};
+            if(!skipA
+               && (st = copy_launch<T>(handle, int(k), int(k), Ab, lda,
+                                       BatchPtr<T>{nullptr, Ac, 0, 0}, k, 1))
+                      != rocblas_status_success)
+                return st;
+        }
+        for(int64_t v0 = 0; v0 < nv; v0 += vstep)
+        {
+            const int64_t vc   = std::min(vstep, nv - v0);
+            BatchPtr<T>   Bv   = Bb.shifted(sideL ? v0 * ldb : v0);
+            const int     rows = int(sideL ? m : vc), cols = int(sideL ? vc : n);
+            BatchPtr<T>   Bu   = Bv;
+            int           ldbu = int(std::min(ldb, i32max));
+            if(cpB)
+            {
+                Bu   = BatchPtr<T>{nullptr, Bc, 0, 0};
+                ldbu = rows;
+                if((st = copy_launch<T>(handle, rows, cols,
+                                        BatchPtr<const T>{(const T* const*)Bv.arr, Bv.base,
+                                                          Bv.stride, Bv.offset},
+                                        ldb, Bu, rows, 1))
+                   != rocblas_status_success)
+                    return st;
+            }
+            st = trsm_entry<T>(handle, side, uplo, transA, diag, rows, cols, alpha, Au,
+                               A_null && !cpA, ldau, Bu, false, ldbu, 1);
+            if(st != rocblas_status_success)
+                return st;
+            if(cpB
+               && (st = copy_launch<T>(handle, rows, cols,
+                                       BatchPtr<const T>{nullptr, Bc, 0, 0}, rows, Bv, ldb, 1))
+                      != rocblas_status_success)
+                return st;
+        }
+    }
+    return rocblas_status_success;
+}
+
 // ILP64 forms (library/src/src64/blas3/rocblas_trsm_64.cpp,
 // rocblas_trsm_kernels_64.cpp:31-225): the same checks on 64-bit sizes, a
 // triangle larger than INT32_MAX is invalid_size as in the reference, and the
 // work is handed to the 32-bit path in chunks of right-hand sides and of the
-// batch.  Leading dimensions beyond INT32_MAX are not implemented here.
+// batch; leading dimensions beyond INT32_MAX go through trsm64_compact.
 template <typename T>
 rocblas_status trsm64_entry(rocblas_handle    handle,
                             rocblas_side      side,
@@ -162,7 +270,8 @@ rocblas_status trsm64_entry(rocblas_handle    handle,
     if(k > i32max)
         return rocblas_status_invalid_size; // exceeds practical memory (reference behaviour)
     if(lda > i32max || ldb > i32max)
-        return rocblas_status_not_implemented;
+        return trsm64_compact<T>(handle, side, uplo, transA, diag, m, n, alpha, A, A_null, lda, B,
+                                 B_null, ldb, batch_count);
     // chunks: right-hand-side vectors (side L: columns, side R: rows) and batch
     const int64_t nv    = sideL ? n : m;
     const int64_t vstep = int64_t(1) << 30;

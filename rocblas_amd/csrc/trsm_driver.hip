@@ -93,8 +93,9 @@ __global__ void __launch_bounds__(256) identity_kernel(int n, BatchPtr<T> X, int
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) copy_kernel(int m, int n, BatchPtr<const T> S, int lds,
-                                                    BatchPtr<T> D, int ldd, int batch)
+__global__ void __launch_bounds__(256) copy_kernel(int m, int n, BatchPtr<const T> S,
+                                                    int64_t lds, BatchPtr<T> D, int64_t ldd,
+                                                    int batch)
 {
     const int64_t total = int64_t(m) * n;
     for(int64_t b = blockIdx.y; b < batch; b += gridDim.y)
@@ -122,8 +123,8 @@ rocblas_status identity_launch(rocblas_handle h, int n, BatchPtr<T> X, int ldx, 
 }
 
 template <typename T>
-rocblas_status copy_launch(rocblas_handle h, int m, int n, BatchPtr<const T> S, int lds,
-                           BatchPtr<T> D, int ldd, int batch)
+rocblas_status copy_launch(rocblas_handle h, int m, int n, BatchPtr<const T> S, int64_t lds,
+                           BatchPtr<T> D, int64_t ldd, int batch)
 {
     const int64_t  total = int64_t(m) * n;
     const unsigned bx    = unsigned(std::min<int64_t>((total + 255) / 256, 4096));
@@ -135,10 +136,10 @@ rocblas_status copy_launch(rocblas_handle h, int m, int n, BatchPtr<const T> S, 
 
 template rocblas_status identity_launch<float>(rocblas_handle, int, BatchPtr<float>, int, int);
 template rocblas_status identity_launch<double>(rocblas_handle, int, BatchPtr<double>, int, int);
-template rocblas_status copy_launch<float>(rocblas_handle, int, int, BatchPtr<const float>, int,
-                                           BatchPtr<float>, int, int);
-template rocblas_status copy_launch<double>(rocblas_handle, int, int, BatchPtr<const double>, int,
-                                            BatchPtr<double>, int, int);
+template rocblas_status copy_launch<float>(rocblas_handle, int, int, BatchPtr<const float>,
+                                           int64_t, BatchPtr<float>, int64_t, int);
+template rocblas_status copy_launch<double>(rocblas_handle, int, int, BatchPtr<const double>,
+                                            int64_t, BatchPtr<double>, int64_t, int);
 
 // ---------------------------------------------------------------------------
 // Small-solve launch plan

@@ -2,13 +2,15 @@
 //
 // Checks and quick returns follow library/src/blas3/rocblas_trtri.hpp:31-62
 // (the size query is answered first, as in library/src/blas3/rocblas_trtri.cpp:55-59).
-// The inverse is computed as the triangular solve op(A) X = I on invA, with
-// the same kernels as rocblas_?trsm (small substitution kernels and the
-// recursive MFMA-updated driver): invA := I, then solve in place.  The
-// triangle opposite `uplo` of invA comes out exactly zero ("may be cleared",
-// rocblas-functions.h:19064-19071).  When invA is A itself (in-place call), A
-// is first copied to the handle workspace.
+// The inverse is built in k^3/3 flops (trtri_core below): the diagonal
+// blocks by the inversion kernel of trtri_kernels.hip, the rest by doubling
+// with triangular-masked MFMA GEMMs.  The triangle opposite `uplo` of invA
+// comes out exactly zero ("may be cleared", rocblas-functions.h:19064-19071).
+// When invA is A itself (in-place call), A is first copied to the handle
+// workspace.
 #include "internal.hpp"
+#include "trtri_kernels.hpp"
+#include "device_common.hpp"
 
 #include <new>
 
@@ -65,6 +67,138 @@ rocblas_status trtri_entry(rocblas_handle    handle,
 
 } // namespace
 
+namespace
+{
+// One doubling step over `groups` equal groups [r0, r0 + n1 + n2) spaced
+// `gstep` rows apart (batch instances each, or, for one instance, the groups
+// as the batch): the inverses X11 (n1) and X22 (n2) of the group's diagonal
+// halves are done, and the off-diagonal block is formed as
+//   lower:  X21 = -X22 (A21 X11)      upper:  X12 = -X11 (A12 X22)
+// with the intermediate product held in the group's opposite block of X
+// (cleared afterwards): stored as is when that block is square (n1 == n2),
+// transposed otherwise.  Triangular masks keep both products to the
+// triangles' flops (2 (n1^2 n2 + n1 n2^2) / 2).
+template <typename T>
+rocblas_status trtri_double_step(rocblas_handle    h,
+                                 bool              upper,
+                                 int               r0,
+                                 int               n1,
+                                 int               n2,
+                                 int               groups,
+                                 int64_t           gstep,
+                                 BatchPtr<const T> A,
+                                 int               lda,
+                                 BatchPtr<T>       X,
+                                 int               ldx,
+                                 int               batch)
+{
+    // operand views at element offset `off` of each instance (or of each
+    // group, when the groups are the batch)
+    const bool multi  = groups > 1;
+    auto       cview  = [&](const BatchPtr<const T>& P, int64_t off, int64_t ld) {
+        return multi ? BatchPtr<const T>{nullptr, P.base, gstep * (ld + 1), P.offset + off}
+                     : P.shifted(off);
+    };
+    auto xview = [&](int64_t off) {
+        return multi ? BatchPtr<T>{nullptr, X.base, gstep * (int64_t(ldx) + 1), X.offset + off}
+                     : X.shifted(off);
+    };
+    auto xcview = [&](int64_t off) {
+        const BatchPtr<const T> Xc{(const T* const*)X.arr, X.base, X.stride, X.offset};
+        return cview(Xc, off, ldx);
+    };
+    const int64_t ldx64 = ldx, lda64 = lda;
+    const int64_t d11   = int64_t(r0) * (ldx64 + 1);        // X11 in X
+    const int64_t d22   = int64_t(r0 + n1) * (ldx64 + 1);   // X22 in X
+    const int64_t x21   = int64_t(r0 + n1) + r0 * ldx64;    // lower-left block of X
+    const int64_t x12   = int64_t(r0) + int64_t(r0 + n1) * ldx64; // upper-right block of X
+    const bool    sq    = n1 == n2;
+
+    GemmArgs<T> g{};
+    g.beta         = T(0);
+    g.skip_if_zero = nullptr;
+    g.batch        = multi ? groups : batch;
+    g.lda          = lda;
+    g.ldb          = ldx;
+    g.ldc          = ldx;
+    // product 1: W = A21 X11 (n2 x n1) or W = A12 X22 (n1 x n2), into the
+    // opposite block; when that block has the other shape, W^T is stored
+    g.alpha = T(1);
+    if(!upper)
+    {
+        const int64_t a21 = int64_t(r0 + n1) + r0 * lda64;
+        if(sq)
+        { // W (n2 x n1) = A21 (n2 x n1) * X11 (n1 x n1, lower)
+            g.M = n2, g.N = n1, g.K = n1, g.opA = 0, g.opB = 0;
+            g.A = cview(A, a21, lda), g.lda = lda;
+            g.B = xcview(d11), g.ldb = ldx;
+            g.tri = GT_B_LOWER;
+        }
+        else
+        { // W^T (n1 x n2) = X11^T (upper) * A21^T
+            g.M = n1, g.N = n2, g.K = n1, g.opA = 1, g.opB = 1;
+            g.A = xcview(d11), g.lda = ldx;
+            g.B = cview(A, a21, lda), g.ldb = lda;
+            g.tri = GT_A_UPPER;
+        }
+        g.C = xview(x12);
+    }
+    else
+    {
+        const int64_t a12 = int64_t(r0) + int64_t(r0 + n1) * lda64;
+        if(sq)
+        { // W (n1 x n2) = A12 (n1 x n2) * X22 (n2 x n2, upper)
+            g.M = n1, g.N = n2, g.K = n2, g.opA = 0, g.opB = 0;
+            g.A = cview(A, a12, lda), g.lda = lda;
+            g.B = xcview(d22), g.ldb = ldx;
+            g.tri = GT_B_UPPER;
+        }
+        else
+        { // W^T (n2 x n1) = X22^T (lower) * A12^T
+            g.M = n2, g.N = n1, g.K = n2, g.opA = 1, g.opB = 1;
+            g.A = xcview(d22), g.lda = ldx;
+            g.B = cview(A, a12, lda), g.ldb = lda;
+            g.tri = GT_A_LOWER;
+        }
+        g.C = xview(x21);
+    }
+    rocblas_status st = launch_gemm<T, T, T>(h, g);
+    if(st != rocblas_status_success)
+        return st;
+
+    // product 2: X21 = -X22 W (n2 x n1) or X12 = -X11 W (n1 x n2)
+    g.alpha = T(-1);
+    g.lda   = ldx;
+    g.ldb   = ldx;
+    g.opA   = 0;
+    g.opB   = sq ? 0 : 1;
+    if(!upper)
+    {
+        g.M = n2, g.N = n1, g.K = n2;
+        g.A   = xcview(d22);
+        g.B   = xcview(x12);
+        g.tri = GT_A_LOWER;
+        g.C   = xview(x21);
+    }
+    else
+    {
+        g.M = n1, g.N = n2, g.K = n1;
+        g.A   = xcview(d11);
+        g.B   = xcview(x21);
+        g.tri = GT_A_UPPER;
+        g.C   = xview(x12);
+    }
+    return launch_gemm<T, T, T>(h, g);
+}
+} // namespace
+
+// Blocked inversion in k^3/3 flops: the diagonal blocks of order kTrtriNB
+// are inverted by trtri_diag_kernel in one launch, then the inverse is built
+// bottom-up by doubling: at level s the groups of 2s rows (the last may be
+// shorter) get their off-diagonal block from two triangular-masked GEMMs
+// (trtri_double_step), all equal groups of a level in one batched launch
+// when there is one instance.  Finally the opposite triangle, used as
+// scratch by the doubling, is cleared.
 template <typename T>
 rocblas_status trtri_core(rocblas_handle    handle,
                           rocblas_fill      uplo,
@@ -89,25 +223,48 @@ rocblas_status trtri_core(rocblas_handle    handle,
         ldas = n;
     }
 
-    rocblas_status st = identity_launch<T>(handle, n, invA, ldinvA, batch_count);
-    if(st != rocblas_status_success)
+    const bool      upper = uplo == rocblas_fill_upper;
+    TrtriDiagArgs<T> d{};
+    d.A     = Asrc;
+    d.lda   = ldas;
+    d.X     = invA;
+    d.ldx   = ldinvA;
+    d.n     = n;
+    d.nb    = n <= 16 ? 16 : kTrtriNB;
+    d.upper = upper;
+    d.unit  = diag == rocblas_diagonal_unit;
+    d.batch = batch_count;
+    rocblas_status st = trtri_diag_launch<T>(handle, d);
+    if(st != rocblas_status_success || n <= d.nb)
         return st;
 
-    TrsmProblem<T> p;
-    p.side       = rocblas_side_left;
-    p.uplo       = uplo;
-    p.trans      = rocblas_operation_none;
-    p.diag       = diag;
-    p.m          = n;
-    p.n          = n;
-    p.lda        = ldas;
-    p.ldb        = ldinvA;
-    p.A          = Asrc;
-    p.B          = invA;
-    p.batch      = batch_count;
-    p.alpha_host = T(1);
-    p.alpha_dev  = nullptr;
-    return trsm_launch<T>(handle, p);
+    for(int64_t s = d.nb; s < n; s *= 2)
+    {
+        const int64_t span   = 2 * s;
+        const int64_t full   = n / span;              // groups with n2 == s
+        const int64_t r_last = full * span;           // start of a shorter group
+        const int64_t n2last = n - r_last - s;        // its n2 (> 0 if it exists)
+        if(full > 0)
+        {
+            if(batch_count == 1 && full > 1 && !Asrc.arr && !invA.arr)
+                st = trtri_double_step<T>(handle, upper, 0, int(s), int(s), int(full), span, Asrc,
+                                          ldas, invA, ldinvA, 1);
+            else
+                for(int64_t gi = 0; gi < full && st == rocblas_status_success; ++gi)
+                    st = trtri_double_step<T>(handle, upper, int(gi * span), int(s), int(s), 1,
+                                              span, Asrc, ldas, invA, ldinvA, batch_count);
+            if(st != rocblas_status_success)
+                return st;
+        }
+        if(n2last > 0)
+        {
+            st = trtri_double_step<T>(handle, upper, int(r_last), int(s), int(n2last), 1, span,
+                                      Asrc, ldas, invA, ldinvA, batch_count);
+            if(st != rocblas_status_success)
+                return st;
+        }
+    }
+    return trtri_zero_opposite_launch<T>(handle, d);
 }
 
 template rocblas_status trtri_core<float>(rocblas_handle, rocblas_fill, rocblas_diagonal, int,

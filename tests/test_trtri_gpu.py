@@ -168,3 +168,64 @@ def test_float32(h):
         G = host(dI, (n, n))
         R = tri_of(ref, uplo, n, False)
         assert np.abs(G - R).max() / np.abs(R).max() <= EPS32 * max(1000.0, n)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_blocked_doubling_sizes(h, uplo, diag):
+    """Orders that exercise the doubling steps of the blocked inversion: equal
+    groups batched in one launch (n = 1024), a shorter last group at several
+    levels (777, 2050), odd leading dimensions; NaN in the opposite triangle
+    of A, a sentinel in invA's padding rows.  Checked against the oracle's
+    xTRTI2 (777) and by ||invA T - I|| on the device (all)."""
+    rng = np.random.default_rng(53)
+    for n in (777, 1024, 2050):
+        say("trtri blocked", uplo, diag, n)
+        A = oracle.diag_dominant_triangular(n, uplo, rng, lda=n + 1)
+        if diag == "U":
+            oracle.make_unit_diagonal(A, uplo, n)
+        dA = dev(A)
+        dI = torch.full(((n + 3) * n,), 7.0, dtype=torch.float64, device="cuda")
+        st = rb.trtri(h, uplo, diag, n, dA.data_ptr(), n + 1, dI.data_ptr(), n + 3)
+        assert st == S.success
+        torch.cuda.synchronize()
+        got = host(dI, (n + 3, n))
+        G = got[:n, :n]
+        off = np.triu(np.ones((n, n), bool), 1) if uplo == "L" else np.tril(np.ones((n, n), bool), -1)
+        assert np.all(G[off] == 0.0)
+        assert np.all(got[n:, :] == 7.0)
+        T = op_dense(A, uplo, "N", diag, n)
+        Gt = torch.from_numpy(np.ascontiguousarray(G)).cuda()
+        Tt = torch.from_numpy(np.ascontiguousarray(T)).cuda()
+        err = float((Gt @ Tt - torch.eye(n, dtype=torch.float64, device="cuda")).abs().max())
+        assert err <= 1e-12, (n, err)
+        if n == 777:
+            ref = A.copy(order="F")
+            oracle.trtri(uplo, diag, n, ref)
+            R = tri_of(ref, uplo, n, diag == "U")
+            assert np.abs(G - R).max() / max(1.0, np.abs(R).max()) <= EPS64 * max(10000.0, n)
+
+
+def test_blocked_batched_matches_single(h):
+    """Strided and pointer-array batches of order 300 (two doubling levels per
+    instance, instances as the GEMM batch) equal the single-instance calls."""
+    rng = np.random.default_rng(59)
+    n, batch = 300, 3
+    for uplo in "LU":
+        As = [oracle.diag_dominant_triangular(n, uplo, rng, poison=0.0) for _ in range(batch)]
+        singles = []
+        for A in As:
+            dA = dev(A)
+            dI = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+            assert rb.trtri(h, uplo, "N", n, dA.data_ptr(), n, dI.data_ptr(), n) == S.success
+            singles.append(dI)
+        say("trtri blocked batched", uplo)
+        flatA = np.concatenate([A.ravel(order="F") for A in As])
+        dAs = torch.from_numpy(flatA).cuda()
+        dIs = torch.zeros(n * n * batch, dtype=torch.float64, device="cuda")
+        assert rb.trtri_strided_batched(h, uplo, "N", n, dAs.data_ptr(), n, n * n, dIs.data_ptr(),
+                                        n, n * n, batch) == S.success
+        torch.cuda.synchronize()
+        for b in range(batch):
+            got = dIs[b * n * n:(b + 1) * n * n]
+            assert torch.allclose(got, singles[b], rtol=0, atol=1e-13), b
