@@ -67,6 +67,34 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_dtrsm_vbatched(rocblas_handle     
                                                              const rocblas_int*  ldb,
                                                              rocblas_int         batch_count);
 
+/* rocblas_amd_dtrsm_vbatched that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, checks and results (bit for bit) as rocblas_amd_dtrsm_vbatched,
+ * but the planning scratch (64 + 36 * batch_count bytes) lives in a
+ * library-owned device array of 2,359,360 bytes per GPU, held until the
+ * process ends, instead of the handle workspace: batch_count may be at most
+ * 65536 (else rocblas_status_invalid_size), and a size query answers
+ * rocblas_status_size_unchanged.  The first call on a device looks up the
+ * array's address there; no call allocates, frees or touches the handle
+ * workspace.  Every call on one GPU shares that array and each call rewrites
+ * it before use, so calls whose executions overlap on one GPU (other
+ * streams, other handles, concurrent graph replays) race and must be
+ * serialised by the caller. */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_dtrsm_vbatched_noalloc(rocblas_handle      handle,
+                                                                     rocblas_side        side,
+                                                                     rocblas_fill        uplo,
+                                                                     rocblas_operation   transA,
+                                                                     rocblas_diagonal    diag,
+                                                                     const rocblas_int*  m,
+                                                                     const rocblas_int*  n,
+                                                                     const double*       alpha,
+                                                                     const double* const A[],
+                                                                     const rocblas_int*  lda,
+                                                                     double* const       B[],
+                                                                     const rocblas_int*  ldb,
+                                                                     rocblas_int batch_count);
+
 /* C := alpha op(A) op(B) + beta C with the MFMA kernel that carries the
  * trailing updates of the blocked solve.  Same signature and argument rules
  * as rocblas_{s,d}gemm (reference rocblas-functions.h:20047-20060); beta == 0

@@ -141,6 +141,7 @@ SYMBOLS = [
     # include/rocblas_amd_ext.h
     ("rocblas_amd_strsm_vbatched", _i32, _TRSM_VBATCHED),
     ("rocblas_amd_dtrsm_vbatched", _i32, _TRSM_VBATCHED),
+    ("rocblas_amd_dtrsm_vbatched_noalloc", _i32, _TRSM_VBATCHED),
     ("rocblas_amd_vbatched_set_tuning", _i32, [_i32, _i32, _i32]),
     ("rocblas_amd_sgemm", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp,
                                  _vp, _i32]),
@@ -316,6 +317,23 @@ def trsm_vbatched(handle, side, uplo, trans, diag, m_arr, n_arr, alpha, A_array,
     st = fn(_h(handle), int(SIDE.get(side, side)), int(FILL.get(uplo, uplo)),
             int(OP.get(trans, trans)), int(DIAG.get(diag, diag)), m_arr, n_arr, a, A_array,
             lda_arr, B_array, ldb_arr, batch)
+    del keep
+    return rocblas_status(st)
+
+
+def trsm_vbatched_noalloc(handle, side, uplo, trans, diag, m_arr, n_arr, alpha, A_array, lda_arr,
+                          B_array, ldb_arr, batch, alpha_ptr=None):
+    """rocblas_amd_dtrsm_vbatched_noalloc: rocblas_amd_dtrsm_vbatched with its
+    planning scratch in a library-owned device array (batch <= 65536)."""
+    a = alpha_ptr
+    keep = None
+    if a is None and alpha is not None:
+        keep = ctypes.c_double(alpha)
+        a = ctypes.addressof(keep)
+    st = lib().rocblas_amd_dtrsm_vbatched_noalloc(
+        _h(handle), int(SIDE.get(side, side)), int(FILL.get(uplo, uplo)),
+        int(OP.get(trans, trans)), int(DIAG.get(diag, diag)), m_arr, n_arr, a, A_array, lda_arr,
+        B_array, ldb_arr, batch)
     del keep
     return rocblas_status(st)
 

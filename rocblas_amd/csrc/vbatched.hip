@@ -35,6 +35,8 @@
 
 #include "../../include/rocblas_amd_ext.h"
 
+#include <atomic>
+#include <mutex>
 #include <new>
 
 namespace rbamd
@@ -395,10 +397,68 @@ rocblas_status vbatched_launch(rocblas_handle h, VbArgs<T> a)
     return launch_status();
 }
 
+// ---------------------------------------------------------------------------
+// Workspace that needs no allocation (rocblas_amd_dtrsm_vbatched_noalloc):
+// the plan's counters, large-instance list and work items for up to
+// kVbNoAllocMaxBatch instances, in a file-scope device array.  Host code
+// never passes the array's name to a kernel: its address on each device is
+// looked up once (hipGetSymbolAddress, with that device current) and kept.
+// ---------------------------------------------------------------------------
+constexpr int    kVbNoAllocMaxBatch = 65536;
+constexpr size_t kVbScratchBytes    = 64 + size_t(kVbNoAllocMaxBatch) * sizeof(int)
+                                   + size_t(kVbNoAllocMaxBatch) * 8 * sizeof(int);
+static_assert(kVbScratchBytes % 64 == 0, "scratch sections are 64-byte rounded");
+
+static __device__ __align__(256) unsigned char scratch[kVbScratchBytes];
+
+constexpr int               kVbMaxDevices = 64;
+static std::atomic<void*>   g_scratch_ptr[kVbMaxDevices];
+static std::mutex           g_scratch_mutex;
+
+// Device address of `scratch` on `device` (current when called), or null.
+static void* vb_scratch_ptr(int device)
+{
+    if(device < 0 || device >= kVbMaxDevices)
+        return nullptr;
+    void* p = g_scratch_ptr[device].load(std::memory_order_acquire);
+    if(p)
+        return p;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    p = g_scratch_ptr[device].load(std::memory_order_relaxed);
+    if(!p)
+    {
+        if(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+        {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_ptr[device].store(p, std::memory_order_release);
+    }
+    return p;
+}
+
 namespace
 {
+// Workspace bytes of a batch: counters, then the large-instance list, then
+// the panel work items (8 per instance).
+inline size_t vb_workspace_bytes(rocblas_int batch_count)
+{
+    return 64 + roundup_ws(size_t(batch_count) * sizeof(int))
+           + roundup_ws(size_t(batch_count) * 8 * sizeof(int));
+}
+
 template <typename T>
-rocblas_status vb_entry(rocblas_handle     handle,
+void vb_carve(VbArgs<T>& a, void* wsp)
+{
+    a.counter = static_cast<int*>(wsp);
+    a.order   = reinterpret_cast<int*>(static_cast<char*>(wsp) + 64);
+    a.items   = reinterpret_cast<int*>(static_cast<char*>(wsp) + 64
+                                     + roundup_ws(size_t(a.batch) * sizeof(int)));
+}
+
+template <typename T>
+rocblas_status vb_entry(bool               noalloc,
+                        rocblas_handle     handle,
                         rocblas_side       side,
                         rocblas_fill       uplo,
                         rocblas_operation  transA,
@@ -425,11 +485,12 @@ rocblas_status vb_entry(rocblas_handle     handle,
         return rocblas_status_invalid_value;
     if(batch_count < 0 || batch_count >= (1 << 28))
         return rocblas_status_invalid_size; // work items encode instance * 8 + group
-    const size_t ws = 64 + roundup_ws(size_t(batch_count) * sizeof(int))
-                      + roundup_ws(size_t(batch_count) * 8 * sizeof(int));
+    if(noalloc && batch_count > kVbNoAllocMaxBatch)
+        return rocblas_status_invalid_size; // beyond the static scratch array
+    const size_t ws = vb_workspace_bytes(batch_count);
     if(handle->size_query)
     {
-        if(!batch_count || roundup_ws(ws) <= handle->query_size)
+        if(noalloc || !batch_count || roundup_ws(ws) <= handle->query_size)
             return rocblas_status_size_unchanged;
         handle->query_size = roundup_ws(ws);
         return rocblas_status_size_increased;
@@ -440,10 +501,10 @@ rocblas_status vb_entry(rocblas_handle     handle,
         return rocblas_status_invalid_pointer;
 
     DeviceGuard    dg(handle->device);
-    WorkspaceLease lease(handle, ws);
-    void*          wsp = lease.ptr;
+    WorkspaceLease lease(handle, noalloc ? 0 : ws);
+    void*          wsp = noalloc ? vb_scratch_ptr(handle->device) : lease.ptr;
     if(!wsp)
-        return rocblas_status_memory_error;
+        return noalloc ? rocblas_status_internal_error : rocblas_status_memory_error;
 
     VbArgs<T> a;
     a.m          = m;
@@ -464,12 +525,7 @@ rocblas_status vb_entry(rocblas_handle     handle,
         a.alpha_dev = alpha;
     a.nb16_max_k = g_vb_tuning.nb16_max_k;
     a.groups     = g_vb_tuning.groups;
-    // counters, then the large-instance list, then the panel work items
-    const size_t order_ints = size_t(batch_count);
-    a.counter    = static_cast<int*>(wsp);
-    a.order      = reinterpret_cast<int*>(static_cast<char*>(wsp) + 64);
-    a.items      = reinterpret_cast<int*>(static_cast<char*>(wsp) + 64
-                                          + roundup_ws(order_ints * sizeof(int)));
+    vb_carve(a, wsp);
     return vbatched_launch<T>(handle, a);
 }
 } // namespace
@@ -502,8 +558,7 @@ rocblas_status vbatched_cases(rocblas_handle     handle,
         return rocblas_status_invalid_size;
     if(!batch_count)
         return rocblas_status_success;
-    const size_t ws = 64 + roundup_ws(size_t(batch_count) * sizeof(int))
-                      + roundup_ws(size_t(batch_count) * 8 * sizeof(int));
+    const size_t ws = vb_workspace_bytes(batch_count);
     DeviceGuard    dg(handle->device);
     WorkspaceLease lease(handle, ws);
     if(!lease.ptr)
@@ -523,10 +578,7 @@ rocblas_status vbatched_cases(rocblas_handle     handle,
     a.alpha_arr  = alpha;
     a.nb16_max_k = g_vb_tuning.nb16_max_k;
     a.groups     = g_vb_tuning.groups;
-    a.counter    = static_cast<int*>(lease.ptr);
-    a.order      = reinterpret_cast<int*>(static_cast<char*>(lease.ptr) + 64);
-    a.items      = reinterpret_cast<int*>(static_cast<char*>(lease.ptr) + 64
-                                          + roundup_ws(size_t(batch_count) * sizeof(int)));
+    vb_carve(a, lease.ptr);
     return vbatched_launch<T>(handle, a);
 }
 
@@ -544,7 +596,7 @@ template rocblas_status vbatched_cases<double>(rocblas_handle, const rocblas_int
 
 using namespace rbamd;
 
-#define RBAMD_VB(NAME, T)                                                                     \
+#define RBAMD_VB(NAME, T, NOALLOC)                                                            \
     rocblas_status NAME(rocblas_handle     handle,                                            \
                         rocblas_side       side,                                              \
                         rocblas_fill       uplo,                                              \
@@ -560,8 +612,8 @@ using namespace rbamd;
                         rocblas_int        batch_count)                                       \
     try                                                                                       \
     {                                                                                         \
-        return vb_entry<T>(handle, side, uplo, transA, diag, m, n, alpha, A, lda, B, ldb,     \
-                           batch_count);                                                      \
+        return vb_entry<T>(NOALLOC, handle, side, uplo, transA, diag, m, n, alpha, A, lda, B, \
+                           ldb, batch_count);                                                 \
     }                                                                                         \
     catch(const std::bad_alloc&)                                                              \
     {                                                                                         \
@@ -573,8 +625,9 @@ using namespace rbamd;
     }
 
 extern "C" {
-RBAMD_VB(rocblas_amd_strsm_vbatched, float)
-RBAMD_VB(rocblas_amd_dtrsm_vbatched, double)
+RBAMD_VB(rocblas_amd_strsm_vbatched, float, false)
+RBAMD_VB(rocblas_amd_dtrsm_vbatched, double, false)
+RBAMD_VB(rocblas_amd_dtrsm_vbatched_noalloc, double, true)
 
 rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb16_max_k,
                                                rocblas_int waves_per_cu,
