@@ -9,8 +9,10 @@ resident in HBM before timing; each step solves in place (eight launches).
 Printed JSON (one line): value = solves per second over the timed steps;
 roofline = the side-left kernel's algorithmic HBM bytes (8*(k(k+1)/2 + 2mn) per
 instance) over its average launch duration measured with HIP events on the
-launch stream; cpu_baseline = the oracle's netlib loop nest, OpenMP over
-instances, on this host's cores (a bounded sample of the same workload).
+launch stream, beside the HBM rate a streaming kernel reaches for the same
+read/write mix on this device (roofline.measured_ceiling); cpu_baseline = the
+oracle's netlib loop nest, OpenMP over instances, on this host's cores (a
+bounded sample of the same workload).
 
 Further workloads (single 512x256 solve, 16384^2 plain and mixed-precision
 solves, a 4096-instance variable-size batch, the Poisson serving loop) are
@@ -148,6 +150,20 @@ def check_accuracy(torch, rb, h, data, Bsrc):
     return worst
 
 
+def hbm_ceiling(torch, rb, h, units=24):
+    """The HBM rate reachable for the side-left launch's read/write mix (about
+    5:2 fetched to written): rocblas_amd_hbm_stream over 5 x units access
+    units read and 2 x units written (about 1 GB and 0.4 GB), 10 passes."""
+    step = rb.hbm_stream_step_bytes(h)
+    rd, wr = 5 * units * step, 2 * units * step
+    src = torch.ones(rd // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty(wr // 4, dtype=torch.int32, device="cuda")
+    sec = rb.hbm_stream(h, src.data_ptr(), rd, dst.data_ptr(), wr, iters=10)
+    del src, dst
+    return {"value": (rd + wr) / sec / 1e9, "unit": "GB/s", "mix": "read:write 5:2",
+            "probe": "rocblas_amd_hbm_stream (16 B per lane per access, 8 waves per SIMD)"}
+
+
 def headline_ratios(work):
     """The metric's comparisons with the plain FP64 solve, lifted from the
     workloads: the precision form over plain FP64 (TFLOP/s at 16384^2), the
@@ -268,6 +284,7 @@ def main():
     achieved = BATCH * algo_bytes("L") / left_avg_s / 1e9
 
     eta = check_accuracy(torch, rb, h, data, Bsrc)
+    ceiling = hbm_ceiling(torch, rb, h)
     if args.copy_calibration:
         # read 124,160,000 B + write 124,160,000 B
         src = torch.ones(BATCH * algo_bytes("L") // 16, dtype=torch.float64, device="cuda")
@@ -310,8 +327,13 @@ def main():
                      "kernel": "small_trsm_dpp_kernel<double,32,*,*,cols> (the 4 side-left cases)",
                      "algorithmic_bytes_per_launch": BATCH * algo_bytes("L"),
                      "avg_launch_us": float(left_us.mean()),
-                     "timing": "HIP events on the launch stream around 4 back-to-back launches"},
+                     "timing": "HIP events on the launch stream around 4 back-to-back launches",
+                     "measured_ceiling": ceiling},
     }
+    if traffic and ceiling:
+        rate = traffic / left_avg_s / 1e9
+        out["roofline"]["traffic_rate"] = rate
+        out["roofline"]["traffic_frac_of_measured_ceiling"] = rate / ceiling["value"]
     if not args.no_vendor:
         v = vendor_strided(torch, data)
         if v is not None:

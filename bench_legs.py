@@ -136,21 +136,22 @@ def leg_large(torch, rb, oracle, vendor, m=16384, reps=3):
     res["plain_fp64"] = {"ms": t * 1e3, "tflops": flops / t / 1e12,
                          "frac_fp64_peak": flops / t / 1e12 / FP64_PEAK_TF, "backward_error": e}
     info_box = {}
+    # the precision form with each update form (tuned block size)
+    for form in ("fp32", "bf16x3"):
+        def mixed(form=form):
+            st, info = rb.trsm_mixed(h, "L", "U", "N", "N", m, n, 1.0, dA.data_ptr(), m,
+                                     dB.data_ptr(), m, max_refine=10, tol=5e-13, form=form)
+            rb.check(st)
+            info_box["info"] = info
 
-    def mixed():
-        st, info = rb.trsm_mixed(h, "L", "U", "N", "N", m, n, 1.0, dA.data_ptr(), m,
-                                 dB.data_ptr(), m, max_refine=10, tol=5e-13)
-        rb.check(st)
-        info_box["info"] = info
-
-    t, _ = timed(mixed)
-    e = eta_device(torch, "L", "U", Am, B0m, dB.view(n, m).T)
-    info = info_box["info"]
-    res["mixed_fp32_refined"] = {"ms": t * 1e3, "tflops": flops / t / 1e12,
-                                 "speedup_vs_plain": res["plain_fp64"]["ms"] / (t * 1e3),
-                                 "refinement_steps": info.iterations,
-                                 "reported_backward_error": info.backward_error,
-                                 "backward_error": e}
+        t, _ = timed(mixed)
+        e = eta_device(torch, "L", "U", Am, B0m, dB.view(n, m).T)
+        info = info_box["info"]
+        res[f"mixed_{form}_refined"] = {"ms": t * 1e3, "tflops": flops / t / 1e12,
+                                        "speedup_vs_plain": res["plain_fp64"]["ms"] / (t * 1e3),
+                                        "refinement_steps": info.iterations,
+                                        "reported_backward_error": info.backward_error,
+                                        "backward_error": e}
     if vendor is not None:
         V, vh = vendor
         one = ctypes.c_double(1.0)

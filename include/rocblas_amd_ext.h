@@ -133,7 +133,10 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_dgemm(rocblas_handle    handle,
 typedef enum rocblas_amd_precision_form_
 {
     /* FP32 copy of A, FP32 MFMA products accumulated in FP32 */
-    rocblas_amd_form_fp32 = 0
+    rocblas_amd_form_fp32 = 0,
+    /* split-BF16x3: each operand value split into BF16 halves hi + lo, the
+     * product hi*hi + hi*lo + lo*hi on BF16 MFMA accumulated in FP32 */
+    rocblas_amd_form_bf16x3 = 1
 } rocblas_amd_precision_form;
 
 /* Outcome of rocblas_amd_dtrsm_mixed (host struct, optional). */
@@ -148,9 +151,10 @@ typedef struct rocblas_amd_refine_info_
 /* FP64 triangular solve, same problem and argument rules as rocblas_dtrsm,
  * computed with low-precision trailing updates and FP64 iterative refinement:
  * FP64 inverses of the block x block diagonal blocks of A (trtri), trailing
- * updates on FP32 MFMA, FP64 residual alpha B - op(A) X, and corrections
+ * updates on FP32 MFMA (form rocblas_amd_form_fp32) or split-BF16x3 MFMA
+ * (rocblas_amd_form_bf16x3), FP64 residual alpha B - op(A) X, and corrections
  * until the normwise backward error is <= tol or max_refine steps are taken.
- * block = 0 picks the tuned block size.  tol <= 0 runs exactly max_refine
+ * block = 0 picks the block size tuned for the form.  tol <= 0 runs exactly max_refine
  * steps without measuring (no host synchronisation); tol > 0 synchronises
  * the stream once per step to test convergence.  Workspace (library-managed
  * handles grow on demand): 2 m n + k^2/2 + nblocks block^2 doubles.
@@ -309,6 +313,15 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_result(rocblas_amd_server s
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb16_max_k,
                                                                   rocblas_int waves_per_cu,
                                                                   rocblas_int panel_groups);
+
+/* Per-size-bucket override of the same table: instances with k in bucket
+ * (0: k <= 64, 1: <= 128, 2: <= 256, 3: <= 512) are split into panel_groups
+ * work items (1..8; 0 restores the built-in value).  rocblas_amd_vbatched_set_tuning
+ * with panel_groups != 0 sets every bucket.  Used by the autotuner; not
+ * thread-safe against concurrent _vbatched calls.  No counterpart in the
+ * reference. */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_vbatched_set_bucket_tuning(rocblas_int bucket,
+                                                                         rocblas_int panel_groups);
 
 /* Completes the handle's stream (skipped while the stream is being captured)
  * and reports whether a kernel of an earlier call on this handle gave up one

@@ -143,6 +143,7 @@ SYMBOLS = [
     ("rocblas_amd_dtrsm_vbatched", _i32, _TRSM_VBATCHED),
     ("rocblas_amd_dtrsm_vbatched_noalloc", _i32, _TRSM_VBATCHED),
     ("rocblas_amd_vbatched_set_tuning", _i32, [_i32, _i32, _i32]),
+    ("rocblas_amd_vbatched_set_bucket_tuning", _i32, [_i32, _i32]),
     ("rocblas_amd_sgemm", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp,
                                  _vp, _i32]),
     ("rocblas_amd_dgemm", _i32, [_vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp,
@@ -354,10 +355,13 @@ class RefineInfo(ctypes.Structure):
                 ("converged", ctypes.c_int32)]
 
 
+MIXED_FORMS = {"fp32": 0, "bf16x3": 1}
+
+
 def trsm_mixed(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, block=0,
-               max_refine=10, tol=1e-13, alpha_ptr=None):
-    """rocblas_amd_dtrsm_mixed (FP32 trailing updates + FP64 refinement).
-    Returns (status, RefineInfo)."""
+               max_refine=10, tol=1e-13, alpha_ptr=None, form="fp32"):
+    """rocblas_amd_dtrsm_mixed (FP32 or split-BF16x3 trailing updates + FP64
+    refinement).  Returns (status, RefineInfo)."""
     info = RefineInfo()
     keep = None
     a = alpha_ptr
@@ -366,8 +370,9 @@ def trsm_mixed(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, blo
         a = ctypes.addressof(keep)
     st = lib().rocblas_amd_dtrsm_mixed(_h(handle), int(SIDE.get(side, side)),
                                        int(FILL.get(uplo, uplo)), int(OP.get(trans, trans)),
-                                       int(DIAG.get(diag, diag)), m, n, a, A, lda, B, ldb, 0,
-                                       block, max_refine, tol, ctypes.byref(info))
+                                       int(DIAG.get(diag, diag)), m, n, a, A, lda, B, ldb,
+                                       MIXED_FORMS.get(form, form), block, max_refine, tol,
+                                       ctypes.byref(info))
     del keep
     return rocblas_status(st), info
 

@@ -587,6 +587,9 @@ def _solve_and_check(handle, arg, call, rng):
         return {}
     out = {"err_host": 0.0, "err_device": 0.0}
     tol = 40 * arg.eps * max(arg.M, 1)
+    inv = _trsm_ex_inverse_blocks(handle, arg, ops)
+    if inv is not None:
+        strides = dict(strides, invA=inv[0].data_ptr(), invA_size=inv[1])
     modes = [m for m, on in ((0, arg.pointer_mode_host), (1, arg.pointer_mode_device)) if on]
     if arg.api in (4, 5):
         modes = [m for m in modes if m == 0] or [0]
@@ -623,6 +626,42 @@ def _solve_and_check(handle, arg, call, rng):
     if arg.timing:
         out.update(time_trsm(handle, arg, call, ops))
     return out
+
+
+TRSM_BLOCK = 128  # the reference clients' diagonal block of invA (testing_trsm_ex.hpp:29)
+
+
+def _trsm_ex_inverse_blocks(handle, arg, ops):
+    """trsm_ex: invA built as the reference's test builds it
+    (clients/include/blas_ex/testing_trsm_ex.hpp:398-440): the inverses of A's
+    TRSM_BLOCK x TRSM_BLOCK diagonal blocks (and of the remainder block) by
+    rocblas_?trtri_strided_batched, passed with invA_size = TRSM_BLOCK * K.
+    The library accepts invA and solves without it; the result must meet the
+    same tolerance.  Returns (device tensor, invA_size) or None."""
+    import torch
+    if arg.function != "trsm_ex" or arg.api != 0:
+        return None
+    K = arg.M if arg.side.upper() == "L" else arg.N
+    if K <= 0:
+        return None
+    p = "s" if arg.precision == "f32_r" else "d"
+    fn = getattr(lib(), f"rocblas_{p}trtri_strided_batched")
+    blocks = K // TRSM_BLOCK
+    dinv = torch.zeros(TRSM_BLOCK * TRSM_BLOCK * (blocks + 1), dtype=ops.dA.dtype, device="cuda")
+    es = ops.dA.element_size()
+    a0 = ops.dA.data_ptr()
+    stride_a = TRSM_BLOCK * arg.lda + TRSM_BLOCK
+    stride_inv = TRSM_BLOCK * TRSM_BLOCK
+    uplo, diag = FILL[arg.uplo.upper()], DIAG[arg.diag.upper()]
+    if blocks > 0:
+        _expect(fn(handle, uplo, diag, TRSM_BLOCK, a0, arg.lda, stride_a, dinv.data_ptr(),
+                   TRSM_BLOCK, stride_inv, blocks), S.success, "trtri of invA blocks")
+    rem = K - TRSM_BLOCK * blocks
+    if rem:
+        _expect(fn(handle, uplo, diag, rem, a0 + stride_a * blocks * es, arg.lda, stride_a,
+                   dinv.data_ptr() + stride_inv * blocks * es, TRSM_BLOCK, stride_inv, 1),
+                S.success, "trtri of the invA remainder")
+    return dinv, TRSM_BLOCK * K
 
 
 def _captured(handle, fn):

@@ -380,9 +380,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     const int wm   = w % 2;
     const int wn   = w / 2;
 
-    // tile index: the same XCD-contiguous, row-panel-grouped order as gemm_kernel
+    // tile index: the same order as gemm_kernel -- XCD-contiguous runs of
+    // row-panel groups, except with a triangular operand, whose row (column)
+    // panels carry unequal work: contiguous runs would give one XCD all the
+    // long panels (1.9x the mean at 16384^2), so dispatch order is kept there
+    // (round-robin over the XCDs)
     const int total = a.tiles_m * a.tiles_n;
     int       bid   = blockIdx.x;
+    if constexpr(!TRI)
     {
         const int q = total / 8, r = total % 8;
         const int x = bid % 8, l = bid / 8;

@@ -20,6 +20,7 @@
 // residual; FP32 carries the trailing updates.  Every returned X has its
 // backward error measured by the residual that accepted it.
 #include "device_common.hpp"
+#include "gemm_bf16x3.hpp"
 #include "mixed_tuning.hpp"
 
 #include "../../include/rocblas_amd_ext.h"
@@ -162,6 +163,7 @@ struct Mixed
     const double*     A;
     int               lda;
     bool              sideL, tA, rev, upperA;
+    bool              bf16x3; // updates as split-BF16x3 products (else FP32 MFMA)
     // workspace
     float*  Alp;   // K x K, ld K
     double* Ainv;  // nslots x NB x NB
@@ -289,7 +291,8 @@ struct Mixed
         return check_launch();
     }
 
-    // Y[I2] -= M[I2, I1] Y[I1] on FP32 MFMA with the FP32 copy of A
+    // Y[I2] -= M[I2, I1] Y[I1] with the FP32 copy of A, on FP32 MFMA or as
+    // split-BF16x3 products (gemm_bf16x3.hip)
     rocblas_status update(double* Y, int ldy, int a1, int b1, int a2, int b2)
     {
         const int     R1 = real_lo(a1, b1), n1 = b1 - a1;
@@ -312,7 +315,7 @@ struct Mixed
             g.ldb   = ldy;
             g.C     = BatchPtr<double>{nullptr, Y, 0, int64_t(R2)};
             g.ldc   = ldy;
-            return launch_gemm(h, g);
+            return bf16x3 ? launch_gemm_bf16x3(h, g) : launch_gemm(h, g);
         }
         GemmArgsT<double, float, double> g{};
         g.alpha = -1.0;
@@ -329,7 +332,7 @@ struct Mixed
         g.ldb   = K;
         g.C     = BatchPtr<double>{nullptr, Y, 0, int64_t(R2) * ldy};
         g.ldc   = ldy;
-        return launch_gemm(h, g);
+        return bf16x3 ? launch_gemm_bf16x3(h, g) : launch_gemm(h, g);
     }
 
     rocblas_status solve(double* Y, int ldy, int a, int b)
@@ -424,12 +427,18 @@ struct Mixed
     }
 };
 
-int tuned_block(int K)
+template <size_t N>
+int table_block(const MixedBlockEntry (&t)[N], int K)
 {
-    for(const auto& e : kMixedBlockTable)
+    for(const auto& e : t)
         if(K <= e.max_k)
             return e.nb;
-    return kMixedBlockTable[sizeof(kMixedBlockTable) / sizeof(kMixedBlockTable[0]) - 1].nb;
+    return t[N - 1].nb;
+}
+
+int tuned_block(int K, bool bf16x3)
+{
+    return bf16x3 ? table_block(kMixedBlockTableBf16x3, K) : table_block(kMixedBlockTableFp32, K);
 }
 
 rocblas_status mixed_entry(rocblas_handle             handle,
@@ -462,7 +471,7 @@ rocblas_status mixed_entry(rocblas_handle             handle,
         return rocblas_status_invalid_value;
     if(diag != rocblas_diagonal_non_unit && diag != rocblas_diagonal_unit)
         return rocblas_status_invalid_value;
-    if(form != rocblas_amd_form_fp32)
+    if(form != rocblas_amd_form_fp32 && form != rocblas_amd_form_bf16x3)
         return rocblas_status_invalid_value;
     const int K = side == rocblas_side_left ? m : n;
     if(m < 0 || n < 0 || lda < K || lda < 1 || ldb < m || ldb < 1)
@@ -512,7 +521,8 @@ rocblas_status mixed_entry(rocblas_handle             handle,
     s.m      = m;
     s.n      = n;
     s.K      = K;
-    s.NB     = std::min(block ? block : tuned_block(K), K);
+    s.bf16x3 = form == rocblas_amd_form_bf16x3;
+    s.NB     = std::min(block ? block : tuned_block(K, s.bf16x3), K);
     s.A      = A;
     s.lda    = lda;
     s.sideL  = side == rocblas_side_left;

@@ -64,21 +64,34 @@ struct VbArgs
     int*               counter; // workspace: [0] panel items taken, [1] large taken,
                                 //            [2] panel items, [3] large instances
     int                nb16_max_k; // wave kernel: k <= this: 16-row blocks, else 32
-    int                groups;     // work items per instance of the panel kernel (1..8)
+    int                groups[kVbPanelBuckets]; // work items per instance of the panel
+                                                // kernel (1..8), by size bucket of k
 };
 
 // Process-wide tuning values; start from the generated table and can be
 // overridden at run time (rocblas_amd_vbatched_set_tuning) by the autotuner.
 struct VbTuning
 {
-    int nb16_max_k      = kVbNb16MaxK;
-    int waves_per_cu    = kVbWavesPerCu;
-    int groups          = kVbPanelGroups;
+    int nb16_max_k       = kVbNb16MaxK;
+    int waves_per_cu     = kVbWavesPerCu;
+    int groups[kVbPanelBuckets];
     int panel_wgs_per_cu = 1;
+    VbTuning()
+    {
+        for(int b = 0; b < kVbPanelBuckets; ++b)
+            groups[b] = kVbPanelGroups[b];
+    }
 };
 static VbTuning g_vb_tuning;
 
 constexpr int VB_BUCKETS = 64;
+
+// size bucket of the panel kernel's tuning table: k <= 64, <= 128, <= 256, <= 512
+__host__ __device__ constexpr int vb_panel_bucket(int k)
+{
+    return k <= 64 ? 0 : k <= 128 ? 1 : k <= 256 ? 2 : 3;
+}
+static_assert(kVbPanelBuckets == 4, "four size buckets up to kPanelMaxK");
 
 __device__ __forceinline__ int vb_cost_bucket(long long k, long long nv)
 {
@@ -111,10 +124,10 @@ __device__ __forceinline__ bool vb_shape(const VbArgs<T>& a, int i, int& k, int&
 }
 
 template <typename T>
-__device__ __forceinline__ int vb_groups(const VbArgs<T>& a, int nv)
+__device__ __forceinline__ int vb_groups(const VbArgs<T>& a, int k, int nv)
 {
     const int panels = (nv + panel::PNC - 1) / panel::PNC;
-    return min(panels, a.groups);
+    return min(panels, a.groups[vb_panel_bucket(k)]);
 }
 
 template <typename T>
@@ -135,7 +148,7 @@ __global__ void __launch_bounds__(1024) vb_plan_kernel(const VbArgs<T> a)
             continue;
         if(k <= kPanelMaxK)
         {
-            const int G   = vb_groups(a, nv);
+            const int G   = vb_groups(a, k, nv);
             const int per = ((nv + panel::PNC - 1) / panel::PNC + G - 1) / G;
             atomicAdd(&hist[vb_cost_bucket(k, min(nv, per * panel::PNC))], G);
         }
@@ -162,7 +175,7 @@ __global__ void __launch_bounds__(1024) vb_plan_kernel(const VbArgs<T> a)
         int k, nv;
         if(!vb_shape(a, i, k, nv) || k > kPanelMaxK)
             continue;
-        const int G   = vb_groups(a, nv);
+        const int G   = vb_groups(a, k, nv);
         const int per = ((nv + panel::PNC - 1) / panel::PNC + G - 1) / G;
         const int pos = atomicAdd(&offs[vb_cost_bucket(k, min(nv, per * panel::PNC))], G);
         for(int g = 0; g < G; ++g)
@@ -172,7 +185,8 @@ __global__ void __launch_bounds__(1024) vb_plan_kernel(const VbArgs<T> a)
 
 // Panel kernel: persistent workgroups over the panel work items.
 template <typename T>
-__global__ void __launch_bounds__(panel::PNT) vb_panel_kernel(const VbArgs<T> a)
+__global__ void __launch_bounds__(panel::PNT)
+    vb_panel_kernel(const VbArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char vb_smem[];
     __shared__ int item_s;
@@ -182,15 +196,21 @@ __global__ void __launch_bounds__(panel::PNT) vb_panel_kernel(const VbArgs<T> a)
         if(threadIdx.x == 0)
             item_s = atomicAdd(&a.counter[0], 1);
         __syncthreads();
-        const int item = item_s;
+        // the work item and the instance's sizes are wave-uniform: keep them
+        // in scalar registers (the panel solve's branches then never write
+        // EXEC, which its DPP updates rely on)
+        const int item = __builtin_amdgcn_readfirstlane(item_s);
         __syncthreads();
         if(item >= total)
             break;
-        const int inst = a.items[item] / 8;
-        const int g    = a.items[item] % 8;
+        const int code = __builtin_amdgcn_readfirstlane(a.items[item]);
+        const int inst = code / 8;
+        const int g    = code % 8;
         int       k, nv;
         vb_shape(a, inst, k, nv);
-        const int G      = vb_groups(a, nv);
+        k  = __builtin_amdgcn_readfirstlane(k);
+        nv = __builtin_amdgcn_readfirstlane(nv);
+        const int G      = vb_groups(a, k, nv);
         const int panels = (nv + panel::PNC - 1) / panel::PNC;
         const T*  Ap     = a.A[inst];
         T*        Bp     = a.B[inst];
@@ -388,7 +408,11 @@ rocblas_status vbatched_launch(rocblas_handle h, VbArgs<T> a)
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     (void)opt;
     (void)hipGetLastError();
-    const int wgs = std::min(a.batch * a.groups, h->num_cus * g_vb_tuning.panel_wgs_per_cu);
+    int gmax = 1;
+    for(int b = 0; b < kVbPanelBuckets; ++b)
+        gmax = std::max(gmax, a.groups[b]);
+    const int wgs = int(std::min<int64_t>(int64_t(a.batch) * gmax,
+                                          int64_t(h->num_cus) * g_vb_tuning.panel_wgs_per_cu));
     hipLaunchKernelGGL(vb_panel_kernel<T>, dim3(wgs), dim3(panel::PNT), lds, h->stream, a);
     if((st = launch_status()) != rocblas_status_success)
         return st;
@@ -524,7 +548,8 @@ rocblas_status vb_entry(bool               noalloc,
     else
         a.alpha_dev = alpha;
     a.nb16_max_k = g_vb_tuning.nb16_max_k;
-    a.groups     = g_vb_tuning.groups;
+    for(int b = 0; b < kVbPanelBuckets; ++b)
+        a.groups[b] = g_vb_tuning.groups[b];
     vb_carve(a, wsp);
     return vbatched_launch<T>(handle, a);
 }
@@ -577,7 +602,8 @@ rocblas_status vbatched_cases(rocblas_handle     handle,
     a.flags_arr  = flags;
     a.alpha_arr  = alpha;
     a.nb16_max_k = g_vb_tuning.nb16_max_k;
-    a.groups     = g_vb_tuning.groups;
+    for(int b = 0; b < kVbPanelBuckets; ++b)
+        a.groups[b] = g_vb_tuning.groups[b];
     vb_carve(a, lease.ptr);
     return vbatched_launch<T>(handle, a);
 }
@@ -629,6 +655,15 @@ RBAMD_VB(rocblas_amd_strsm_vbatched, float, false)
 RBAMD_VB(rocblas_amd_dtrsm_vbatched, double, false)
 RBAMD_VB(rocblas_amd_dtrsm_vbatched_noalloc, double, true)
 
+rocblas_status rocblas_amd_vbatched_set_bucket_tuning(rocblas_int bucket, rocblas_int panel_groups)
+{
+    if(bucket < 0 || bucket >= kVbPanelBuckets || panel_groups < 0 || panel_groups > 8)
+        return rocblas_status_invalid_value;
+    const VbTuning def;
+    g_vb_tuning.groups[bucket] = panel_groups ? panel_groups : def.groups[bucket];
+    return rocblas_status_success;
+}
+
 rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb16_max_k,
                                                rocblas_int waves_per_cu,
                                                rocblas_int panel_groups)
@@ -639,7 +674,8 @@ rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb16_max_k,
     const VbTuning def;
     g_vb_tuning.nb16_max_k   = nb16_max_k ? nb16_max_k : def.nb16_max_k;
     g_vb_tuning.waves_per_cu = waves_per_cu ? waves_per_cu : def.waves_per_cu;
-    g_vb_tuning.groups       = panel_groups ? panel_groups : def.groups;
+    for(int b = 0; b < kVbPanelBuckets; ++b)
+        g_vb_tuning.groups[b] = panel_groups ? panel_groups : def.groups[b];
     return rocblas_status_success;
 }
 }

@@ -1,5 +1,6 @@
-"""GPU tests of rocblas_amd_dtrsm_mixed (FP32 trailing updates, FP64 diagonal-
-block inverses, FP64 residual and iterative refinement) through the C ABI.
+"""GPU tests of rocblas_amd_dtrsm_mixed (FP32 or split-BF16x3 trailing
+updates, FP64 diagonal-block inverses, FP64 residual and iterative
+refinement) through the C ABI.
 
 The mixed form answers the same problem as rocblas_dtrsm
 (library/src/blas3/rocblas_trsm.cpp); results are checked against the oracle
@@ -39,15 +40,16 @@ def eta_np(side, uplo, trans, diag, alpha, A, B0, X):
     return n1(R) / (n1(op) * n1(X) + abs(alpha) * n1(B0))
 
 
+@pytest.mark.parametrize("form", ["fp32", "bf16x3"])
 @pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
 @pytest.mark.parametrize("diag", ["N", "U"])
-def test_against_oracle_and_backward_error(h, side, uplo, trans, diag):
+def test_against_oracle_and_backward_error(h, side, uplo, trans, diag, form):
     """Sizes with full and partial diagonal blocks (block 128): the refined result
     meets eta <= 1e-12, agrees with the oracle, and the reported eta is the
     measured one."""
     rng = np.random.default_rng(200 + 2 * ALL_CASES.index((side, uplo, trans)) + (diag == "U"))
     for m, n in [(300, 70), (256, 129), (90, 333)]:
-        say("mixed", side, uplo, trans, diag, m, n)
+        say("mixed", form, side, uplo, trans, diag, m, n)
         k = m if side == "L" else n
         A = oracle.diag_dominant_triangular(k, uplo, rng, lda=k + 1)
         if diag == "U":
@@ -57,7 +59,8 @@ def test_against_oracle_and_backward_error(h, side, uplo, trans, diag):
         alpha = -0.75
         dA, dB = dev(A), dev(B)
         st, info = rb.trsm_mixed(h, side, uplo, trans, diag, m, n, alpha, dA.data_ptr(), k + 1,
-                                 dB.data_ptr(), m + 2, block=128, max_refine=8, tol=1e-14)
+                                 dB.data_ptr(), m + 2, block=128, max_refine=8, tol=1e-14,
+                                 form=form)
         assert st == S.success
         torch.cuda.synchronize()
         got = host(dB, B.shape)
@@ -166,3 +169,42 @@ def test_reported_eta_through_the_dma_residual(h, side, uplo, trans, diag):
                     logical(dB, m, m, n))
     assert 1e-12 < eta < 1e-3, eta
     assert abs(info.backward_error - eta) <= 1e-6 * eta, (info.backward_error, eta)
+
+
+@pytest.mark.parametrize("side,uplo,trans", [("L", "U", "N"), ("R", "L", "T")])
+def test_first_solve_accuracy_per_form(h, side, uplo, trans):
+    """The unrefined first solve (max_refine 0) carries the update form's
+    accuracy: FP32 products keep the backward error below 2e-6, split-BF16x3
+    products (three BF16 terms, about 16 bits) below 1e-4 and measurably above
+    FP32's -- a single BF16 product (8 bits) would give about 1e-3, so a wrong
+    split or a dropped cross term fails the bound.  Refinement then meets
+    1e-12 within two steps."""
+    rng = np.random.default_rng(31)
+    m, n = (1024, 96) if side == "L" else (96, 1024)
+    k = m if side == "L" else n
+    # diagonal U[1,2], off-diagonal U[-1,1]/k: values that neither FP32 nor
+    # BF16 halves hold exactly (small integers would split without error)
+    A = np.asfortranarray(np.triu(rng.uniform(-1, 1, (k, k)) / k, 1) if uplo == "U"
+                          else np.tril(rng.uniform(-1, 1, (k, k)) / k, -1))
+    A[np.arange(k), np.arange(k)] = rng.uniform(1, 2, k)
+    B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    bounds = {"fp32": 2e-6, "bf16x3": 1e-4}
+    first = {}
+    for form, hi in bounds.items():
+        say("mixed first solve", form, side, uplo, trans)
+        dA, dB = dev(A), dev(B)
+        st, info = rb.trsm_mixed(h, side, uplo, trans, "N", m, n, 1.0, dA.data_ptr(), k,
+                                 dB.data_ptr(), m, block=128, max_refine=0, tol=0.0, form=form)
+        assert st == S.success
+        torch.cuda.synchronize()
+        e0 = eta_np(side, uplo, trans, "N", 1.0, A, B, host(dB, B.shape))
+        assert 0.0 < e0 <= hi, (form, e0)
+        first[form] = e0
+        dB = dev(B)
+        st, info = rb.trsm_mixed(h, side, uplo, trans, "N", m, n, 1.0, dA.data_ptr(), k,
+                                 dB.data_ptr(), m, block=128, max_refine=5, tol=1e-13, form=form)
+        assert st == S.success and info.converged == 1 and info.iterations <= 2
+        torch.cuda.synchronize()
+        assert eta_np(side, uplo, trans, "N", 1.0, A, B, host(dB, B.shape)) <= 1e-12
+    # 16-bit products are measurably coarser than FP32 ones
+    assert first["bf16x3"] > 2.0 * first["fp32"], first

@@ -687,3 +687,39 @@ def test_ilp64_forms(h):
                               dA.data_ptr(), 4, got.data_ptr(), 4) == S.invalid_handle
     assert L.rocblas_dtrsm_64(h.ptr, 141, 122, 111, 131, 0, 1, ctypes.addressof(alpha),
                               None, 4, None, 4) == S.success
+
+
+@pytest.mark.parametrize("precision", ["d", "s"])
+def test_dma_gemm_batch_loop_matches_single(h, precision):
+    """2048 x 2048 solves with batch 2: the recursion's first update is
+    1024 x 2048 x 1024 over two instances, which runs the LDS-DMA GEMM's batch
+    loop (grid y, per-instance strides, LDS reuse across instances).  Each
+    instance of the strided and the pointer-array forms equals its
+    single-instance solve bit for bit."""
+    rng = np.random.default_rng(2048)
+    n, batch = 2048, 2
+    dt = np.float64 if precision == "d" else np.float32
+    tdt = torch.float64 if precision == "d" else torch.float32
+    As = [oracle.diag_dominant_triangular(n, "U", rng, dtype=dt, poison=0.0) for _ in range(batch)]
+    Bs = [np.asfortranarray(rng.uniform(-1, 1, (n, n)).astype(dt)) for _ in range(batch)]
+    singles = []
+    for A, B in zip(As, Bs):
+        dA, dB = dev(A), dev(B)
+        assert rb.trsm(h, "L", "U", "N", "N", n, n, 1.0, dA.data_ptr(), n, dB.data_ptr(), n,
+                       precision) == S.success
+        singles.append(dB)
+    say("dma batch loop", precision)
+    dAs = torch.cat([dev(A) for A in As])
+    dBs = torch.cat([dev(B) for B in Bs])
+    assert rb.trsm_strided_batched(h, "L", "U", "N", "N", n, n, 1.0, dAs.data_ptr(), n, n * n,
+                                   dBs.data_ptr(), n, n * n, batch, precision) == S.success
+    dAl = [dev(A) for A in As]
+    dBl = [dev(B) for B in Bs]
+    pa, pb = ptr_array(dAl), ptr_array(dBl)
+    assert rb.trsm_batched(h, "L", "U", "N", "N", n, n, 1.0, pa.data_ptr(), n, pb.data_ptr(), n,
+                           batch, precision) == S.success
+    torch.cuda.synchronize()
+    for b in range(batch):
+        assert torch.equal(dBs[b * n * n:(b + 1) * n * n], singles[b]), ("strided", b)
+        assert torch.equal(dBl[b], singles[b]), ("pointer array", b)
+    assert singles[0].dtype == tdt
