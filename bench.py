@@ -150,18 +150,20 @@ def check_accuracy(torch, rb, h, data, Bsrc):
     return worst
 
 
-def hbm_ceiling(torch, rb, h, units=24):
+def hbm_ceiling(torch, rb, h, units=12):
     """The HBM rate reachable for the side-left launch's read/write mix (about
     5:2 fetched to written): rocblas_amd_hbm_stream over 5 x units access
-    units read and 2 x units written (about 1 GB and 0.4 GB), 10 passes."""
+    units read and 2 x units written (about 1 GB and 0.4 GB), the best of six
+    launch shapes, 5 passes each."""
     step = rb.hbm_stream_step_bytes(h)
     rd, wr = 5 * units * step, 2 * units * step
     src = torch.ones(rd // 4, dtype=torch.int32, device="cuda")
     dst = torch.empty(wr // 4, dtype=torch.int32, device="cuda")
-    sec = rb.hbm_stream(h, src.data_ptr(), rd, dst.data_ptr(), wr, iters=10)
+    sec = rb.hbm_stream(h, src.data_ptr(), rd, dst.data_ptr(), wr, iters=5)
     del src, dst
     return {"value": (rd + wr) / sec / 1e9, "unit": "GB/s", "mix": "read:write 5:2",
-            "probe": "rocblas_amd_hbm_stream (16 B per lane per access, 8 waves per SIMD)"}
+            "probe": "rocblas_amd_hbm_stream (16 B per lane per access; best of 4/8/16 "
+                     "waves per SIMD, plain and non-temporal)"}
 
 
 def headline_ratios(work):

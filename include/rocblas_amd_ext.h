@@ -343,17 +343,17 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_set_chain_test_timeout(rocblas_han
 
 /* Measures the HBM rate a memory-bound kernel can reach on this device for a
  * given mix of reads and writes: a streaming kernel (16 bytes per lane per
- * access, 8 wavefronts per SIMD, several loads in flight per wavefront) reads
- * read_bytes from src and writes write_bytes to dst, iters times after one
- * untimed launch, on the handle's stream; *seconds is the average time of one
- * pass (the call synchronises on its last event).  read_bytes : write_bytes
+ * access, several loads in flight per wavefront) reads read_bytes from src
+ * and writes write_bytes to dst, iters times after one untimed launch, on the
+ * handle's stream, for each of 4, 8 and 16 wavefronts per SIMD with plain and
+ * with non-temporal loads and stores; *seconds is the best average time of
+ * one pass (the call synchronises on its events).  read_bytes : write_bytes
  * must be one of 1:0, 0:1, 1:1, 2:1, 3:1 or 5:2, and each a whole number of
- * steps (rocblas_amd_hbm_stream_step_bytes() bytes per access unit), else
+ * units (rocblas_amd_hbm_stream_step_bytes() bytes per access unit), else
  * rocblas_status_invalid_size.  With write_bytes == 0, dst must still hold
  * one access unit (the kernel may write there to keep its loads live).  The
- * triangular solves quote their bandwidth
- * against this rate beside the published peak.  No counterpart in the
- * reference. */
+ * triangular solves quote their bandwidth against this rate beside the
+ * published peak.  No counterpart in the reference. */
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_hbm_stream(rocblas_handle handle,
                                                          const void*    src,
                                                          size_t         read_bytes,
@@ -363,7 +363,7 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_hbm_stream(rocblas_handle handle,
                                                          double*        seconds);
 
 /* Bytes one access unit of rocblas_amd_hbm_stream moves per read or write
- * (grid threads x 16). */
+ * (16 bytes for each thread of a grid of 16 wavefronts per SIMD). */
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_hbm_stream_step_bytes(rocblas_handle handle,
                                                                     size_t*        bytes);
 

@@ -7,9 +7,10 @@
 // bandwidth; their roofline fraction is quoted against the 8 TB/s peak, and
 // this kernel gives the attainable rate for the same mix of reads and writes:
 // every lane moves 16 bytes per access (dwordx4), each wavefront keeps
-// UNROLL x (reads per step) loads in flight, and the grid holds 8 wavefronts
-// per SIMD, so the rate it reaches is set by the memory system, not by
-// latency.
+// UNROLL x (reads per step) loads in flight, and the reported time is the
+// best over grids of 4, 8 and 16 wavefronts per SIMD with plain and with
+// non-temporal loads and stores, so the rate is set by the memory system,
+// not by latency or one cache policy.
 //
 // Layout of one step (G = threads in the grid): thread g reads
 // src[(step*RQ + q)*G + g] for q < RQ and writes dst[(step*WQ + q)*G + g]
@@ -29,7 +30,25 @@ namespace
 {
 using u32x4 = unsigned __attribute__((ext_vector_type(4)));
 
-template <int RQ, int WQ, int UNROLL>
+template <bool NT>
+__device__ __forceinline__ u32x4 ld16(const u32x4* p)
+{
+    if constexpr(NT)
+        return __builtin_nontemporal_load((const __attribute__((address_space(1))) u32x4*)(p));
+    else
+        return gload(p);
+}
+
+template <bool NT>
+__device__ __forceinline__ void st16(u32x4* p, u32x4 v)
+{
+    if constexpr(NT)
+        __builtin_nontemporal_store(v, (__attribute__((address_space(1))) u32x4*)(p));
+    else
+        gstore(p, v);
+}
+
+template <int RQ, int WQ, int UNROLL, bool NT>
 __global__ void __launch_bounds__(256) hbm_stream_kernel(const u32x4* __restrict__ src,
                                                          u32x4* __restrict__ dst,
                                                          long long steps,
@@ -47,7 +66,7 @@ __global__ void __launch_bounds__(256) hbm_stream_kernel(const u32x4* __restrict
             const long long s = s0 + u < steps ? s0 + u : steps - 1;
 #pragma unroll
             for(int q = 0; q < RQ; ++q)
-                v[u][q] = gload(src + ((s * RQ + q) * G + g));
+                v[u][q] = ld16<NT>(src + ((s * RQ + q) * G + g));
         }
 #pragma unroll
         for(int u = 0; u < UNROLL; ++u)
@@ -64,7 +83,7 @@ __global__ void __launch_bounds__(256) hbm_stream_kernel(const u32x4* __restrict
                 {
                     u32x4 o = w;
                     o.x += unsigned(q);
-                    gstore(dst + (((s0 + u) * WQ + q) * G + g), o);
+                    st16<NT>(dst + (((s0 + u) * WQ + q) * G + g), o);
                 }
             }
         }
@@ -74,9 +93,10 @@ __global__ void __launch_bounds__(256) hbm_stream_kernel(const u32x4* __restrict
         gstore(sink + g, acc);
 }
 
-template <int RQ, int WQ>
-rocblas_status run_form(rocblas_handle h, const u32x4* src, u32x4* dst, long long steps,
-                        long long G, int iters, double* seconds)
+// one launch shape: G threads, NT cache policy; average seconds of a pass
+template <int RQ, int WQ, bool NT>
+rocblas_status run_shape(rocblas_handle h, const u32x4* src, u32x4* dst, long long steps,
+                         long long G, int iters, double* seconds)
 {
     constexpr int UNROLL = (RQ + WQ) <= 2 ? 8 : 4;
     const dim3    grid(unsigned(G / 256));
@@ -89,14 +109,14 @@ rocblas_status run_form(rocblas_handle h, const u32x4* src, u32x4* dst, long lon
         return rocblas_status_internal_error;
     }
     // one untimed launch brings the code and the TLB entries in
-    hipLaunchKernelGGL((hbm_stream_kernel<RQ, WQ, UNROLL>), grid, dim3(256), 0, h->stream, src,
-                       dst, steps, dst);
+    hipLaunchKernelGGL((hbm_stream_kernel<RQ, WQ, UNROLL, NT>), grid, dim3(256), 0, h->stream,
+                       src, dst, steps, dst);
     rocblas_status st = launch_status();
     if(st == rocblas_status_success)
     {
         (void)hipEventRecord(e0, h->stream);
         for(int i = 0; i < iters; ++i)
-            hipLaunchKernelGGL((hbm_stream_kernel<RQ, WQ, UNROLL>), grid, dim3(256), 0,
+            hipLaunchKernelGGL((hbm_stream_kernel<RQ, WQ, UNROLL, NT>), grid, dim3(256), 0,
                                h->stream, src, dst, steps, dst);
         st = launch_status();
         (void)hipEventRecord(e1, h->stream);
@@ -112,6 +132,33 @@ rocblas_status run_form(rocblas_handle h, const u32x4* src, u32x4* dst, long lon
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return st;
+}
+
+// the best over 4, 8 and 16 wavefronts per SIMD, plain and non-temporal;
+// `unit` is the bytes of one access step at 16 wavefronts per SIMD
+template <int RQ, int WQ>
+rocblas_status run_form(rocblas_handle h, const u32x4* src, u32x4* dst, long long units,
+                        int iters, double* seconds)
+{
+    const long long G16  = (long long)h->num_cus * 4 * 16 * WAVE;
+    double          best = 0.0;
+    for(int wps : {4, 8, 16})
+    {
+        const long long G     = G16 / 16 * wps;
+        const long long steps = units * (16 / wps);
+        for(int nt = 0; nt < 2; ++nt)
+        {
+            double         t  = 0.0;
+            rocblas_status st = nt ? run_shape<RQ, WQ, true>(h, src, dst, steps, G, iters, &t)
+                                   : run_shape<RQ, WQ, false>(h, src, dst, steps, G, iters, &t);
+            if(st != rocblas_status_success)
+                return st;
+            if(best == 0.0 || t < best)
+                best = t;
+        }
+    }
+    *seconds = best;
+    return rocblas_status_success;
 }
 
 } // namespace
@@ -147,24 +194,23 @@ try
     if(f < 0)
         return rocblas_status_invalid_size;
     const int       rq = forms[f].rq, wq = forms[f].wq;
-    const long long G  = (long long)handle->num_cus * 4 * 8 * WAVE; // 8 waves per SIMD
-    const long long per_step = G * 16;
-    const long long steps
-        = rq ? (long long)(read_bytes / (per_step * rq)) : (long long)(write_bytes / (per_step * wq));
-    if(steps < 1 || (rq && size_t(steps * per_step * rq) != read_bytes)
-       || (wq && size_t(steps * per_step * wq) != write_bytes))
-        return rocblas_status_invalid_size; // byte counts must be whole steps
+    const long long per_unit = (long long)handle->num_cus * 4 * 16 * WAVE * 16; // 16 waves/SIMD
+    const long long units
+        = rq ? (long long)(read_bytes / (per_unit * rq)) : (long long)(write_bytes / (per_unit * wq));
+    if(units < 1 || (rq && size_t(units * per_unit * rq) != read_bytes)
+       || (wq && size_t(units * per_unit * wq) != write_bytes))
+        return rocblas_status_invalid_size; // byte counts must be whole units
     DeviceGuard  dg(handle->device);
     const u32x4* s = static_cast<const u32x4*>(src);
     u32x4*       d = static_cast<u32x4*>(dst);
     switch(rq * 10 + wq)
     {
-    case 10: return run_form<1, 0>(handle, s, d, steps, G, iters, seconds);
-    case 1: return run_form<0, 1>(handle, s, d, steps, G, iters, seconds);
-    case 11: return run_form<1, 1>(handle, s, d, steps, G, iters, seconds);
-    case 21: return run_form<2, 1>(handle, s, d, steps, G, iters, seconds);
-    case 52: return run_form<5, 2>(handle, s, d, steps, G, iters, seconds);
-    case 31: return run_form<3, 1>(handle, s, d, steps, G, iters, seconds);
+    case 10: return run_form<1, 0>(handle, s, d, units, iters, seconds);
+    case 1: return run_form<0, 1>(handle, s, d, units, iters, seconds);
+    case 11: return run_form<1, 1>(handle, s, d, units, iters, seconds);
+    case 21: return run_form<2, 1>(handle, s, d, units, iters, seconds);
+    case 52: return run_form<5, 2>(handle, s, d, units, iters, seconds);
+    case 31: return run_form<3, 1>(handle, s, d, units, iters, seconds);
     default: return rocblas_status_invalid_size;
     }
 }
@@ -179,6 +225,6 @@ extern "C" rocblas_status rocblas_amd_hbm_stream_step_bytes(rocblas_handle handl
         return rocblas_status_invalid_handle;
     if(!bytes)
         return rocblas_status_invalid_pointer;
-    *bytes = size_t(handle->num_cus) * 4 * 8 * WAVE * 16;
+    *bytes = size_t(handle->num_cus) * 4 * 16 * WAVE * 16;
     return rocblas_status_success;
 }
