@@ -204,6 +204,10 @@ def main():
                     help="further workloads: all, none, or a comma list of "
                          "single,large,vbatch,serve")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--vbatch-eager", action="store_true",
+                    help="time the variable-size batch as a direct launch instead of a graph "
+                         "replay (for profilers whose counter collection does not follow graph "
+                         "launches)")
     ap.add_argument("--no-vendor", action="store_true",
                     help="do not time the vendor librocblas beside the further workloads")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -358,7 +362,8 @@ def main():
         work = {}
         runs = [("single_512x256", "single", lambda: bl.leg_single(torch, rb, synth, oracle, vendor)),
                 ("large_16384", "large", lambda: bl.leg_large(torch, rb, oracle, vendor)),
-                ("variable_batch_4096", "vbatch", lambda: bl.leg_variable_batch(torch, rb, vendor)),
+                ("variable_batch_4096", "vbatch",
+                 lambda: bl.leg_variable_batch(torch, rb, vendor, eager=args.vbatch_eager)),
                 ("trtri", "trtri", lambda: bl.leg_trtri(torch, rb, vendor)),
                 ("serving_poisson_2000rps", "serve",
                  lambda: bl.leg_serving(torch, rb, duration=args.serve_seconds))]
