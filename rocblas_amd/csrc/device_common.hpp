@@ -48,6 +48,35 @@ struct Mfma<float>
     }
 };
 
+// Wave-uniform copies of a value every lane holds (the compiler then keeps
+// it, and what is derived from it, in scalar registers)
+template <typename P>
+__device__ __forceinline__ P* uniform_ptr(P* p)
+{
+    const uint64_t v  = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<P*>((uint64_t(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ int uniform_int(int v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ double uniform_val(double x)
+{
+    const uint64_t v  = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ float uniform_val(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, x)));
+}
+
 // Explicit global-address-space access: pointers read from LDS tables are
 // generic, and generic (flat) memory instructions also count against the LDS
 // wait counter, which would serialise LDS waits behind HBM loads.

@@ -16,9 +16,9 @@
 // vectors, a 16-lane row per vector, lane g holding rows g, g+16, g+32, g+48,
 // and every update of row i by the solved value of row j is one
 // v_fmac_dpp that broadcasts it from lane j%16 (row_newbcast).  Tiles of L
-// are staged through LDS with coalesced
-// loads (either storage order of A), the next tile's loads in flight during
-// the current tile's MFMAs.
+// are staged through LDS with coalesced loads (either storage order of A),
+// the next two tiles' loads in flight during the current tile's MFMAs or
+// substitution.
 #pragma once
 
 #include "device_common.hpp"
@@ -153,100 +153,122 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
     };
 
     // Tile (I, J) of canonical L, element s of this thread: canonical
-    // (i, j) chosen so that consecutive threads read consecutive addresses.
-    auto tile_ij = [&](int s, int& i, int& j) {
-        const int e = tid + PNT * s;
-        if(!tA)
-        {
-            i = e % PNB; // contiguous real rows (reversed when rev)
-            j = e / PNB;
-        }
-        else
-        {
-            j = e % PNB; // A^T: contiguous along canonical columns
-            i = e / PNB;
-        }
+    // (i, j) = (lane, wu + 4s) (A read as is) or (wu + 4s, lane) (A^T), so
+    // that consecutive lanes read consecutive addresses.  The lane's index
+    // fixes the address within a column of A's storage and the wavefront's
+    // (wu + 4s) picks the column, which is wave-uniform: each element is one
+    // load at a scalar column address plus the lane's fixed offset.
+    const int wu      = __builtin_amdgcn_readfirstlane(w);
+    auto      tile_ij = [&](int s, int& i, int& j) {
+        i = tA ? wu + 4 * s : lane;
+        j = tA ? lane : wu + 4 * s;
     };
-    // raw values of the tile in flight; the triangle mask is applied when the
-    // tile is written to LDS, so no load is predicated on it
-    T    reg[PEPT];
-    int  tI = 0, tJ = 0;
-    auto load_tile = [&](int I, int J) {
-        tI = I;
-        tJ = J;
+    auto fold = [&](int x) { return rev ? K - 1 - x : x; };
+    // Raw values of a tile in flight.  Indices past K - 1 read row (column)
+    // K - 1 instead, so every load is in bounds and unpredicated; only the
+    // last block row has such rows, and their results are never used.
+    auto load_tile = [&](T(&reg)[PEPT], int I, int J) {
+        const int p  = fold(min((tA ? J : I) * PNB + lane, K - 1)); // lane's row in A
+        const int qb = (tA ? I : J) * PNB + wu;
 #pragma unroll
         for(int s = 0; s < PEPT; ++s)
         {
-            int i, j;
-            tile_ij(s, i, j);
-            const int ci = min(I * PNB + i, K - 1), cj = min(J * PNB + j, K - 1);
-            const int mi = rev ? K - 1 - ci : ci, mj = rev ? K - 1 - cj : cj;
-            reg[s] = gload(Ap + (tA ? int64_t(mj) + int64_t(mi) * lda
-                                    : int64_t(mi) + int64_t(mj) * lda));
+            const int q = fold(min(qb + 4 * s, K - 1)); // the column, wave-uniform
+            reg[s]      = gload(Ap + int64_t(q) * lda + p);
         }
     };
-    auto store_tile = [&]() {
+    // An off-diagonal tile (J < I) is dense and its columns are all below K;
+    // rows past K (last block row) carry row K - 1's values into MFMA result
+    // rows that are discarded, so it is stored without a mask.
+    const int lds0 = tA ? wu * PLDA + lane : lane * PLDA + wu; // element 0 of this thread
+    const int ldss = tA ? 4 * PLDA : 4;                        // step between elements
+    auto      store_tile = [&](const T(&reg)[PEPT]) {
 #pragma unroll
         for(int s = 0; s < PEPT; ++s)
-        {
-            int i, j;
-            tile_ij(s, i, j);
-            const int  gi = tI * PNB + i, gj = tJ * PNB + j;
-            const bool in = gi < K && gj <= gi && !(unit && gi == gj);
-            Ls[i * PLDA + j] = in ? reg[s] : T(0);
-        }
+            Ls[lds0 + s * ldss] = reg[s];
     };
     // the diagonal block: -L strictly below the diagonal (the fused
     // substitution's operand), the diagonal into rdiag
-    auto store_diag = [&]() {
+    auto store_diag = [&](const T(&reg)[PEPT], int I) {
 #pragma unroll
         for(int s = 0; s < PEPT; ++s)
         {
             int i, j;
             tile_ij(s, i, j);
-            const int gi = tI * PNB + i;
+            const int gi = I * PNB + i;
             Ls[i * PLDA + j] = (gi < K && j < i) ? -reg[s] : T(0);
             if(i == j)
                 rdiag[i] = (gi < K && !unit) ? reg[s] : T(1);
         }
     };
-
-    // tile sequence: (0,0), (1,0), (1,1), (2,0), ... ; (I, I) is the diagonal block
-    load_tile(0, 0);
-    for(int I = 0; I < nblk; ++I)
-    {
-        // right-hand sides of this block, consumed after the accumulation
-        T bv[4];
+    // right-hand sides of block row I; rows past K and vectors past nvec
+    // read an element in bounds (row K - 1, vector nvec - 1), discarded
+    // later, so the loads need no branch
+    const int vclamp = min(vec, nvec - 1);
+    auto      load_rhs = [&](T(&bv)[4], int I) {
 #pragma unroll
         for(int r = 0; r < 4; ++r)
+            bv[r] = gload(Bp + boff(min(I * PNB + g + 16 * r, K - 1), vclamp));
+    };
+
+    // The tiles are consumed in the order (0,0), (1,0), (1,1), (2,0), ...,
+    // (I, I) being the diagonal block that ends block row I.  Two tiles are
+    // in flight at any time (register buffers ra and rb, used alternately):
+    // a buffer is refilled with the tile two places ahead as soon as its
+    // tile is in LDS, so each tile's load latency overlaps the two steps
+    // before it is needed.
+    // Every step issues exactly one tile's loads (past the last tile, the
+    // last tile again, never consumed), so each path through a step leaves
+    // the same number of loads in flight and the compiler's wait counts
+    // retire only the tile being consumed.
+    int  iI = 0, iJ = 0; // next tile to load
+    auto issue = [&](T(&reg)[PEPT]) {
+        const bool past = iI >= nblk;
+        load_tile(reg, past ? nblk - 1 : iI, past ? nblk - 1 : iJ);
+        if(!past && ++iJ > iI)
         {
-            const int gi = I * PNB + g + 16 * r;
-            bv[r]        = gload(Bp + ((vok && gi < K) ? boff(gi, vec) : 0));
+            ++iI;
+            iJ = 0;
         }
-        // ---- c_I accumulation: sum_{J<I} L_IJ y_J on MFMA (two chains) ----
-        typename Mfma<T>::acc_t acc0{0, 0, 0, 0}, acc1{0, 0, 0, 0};
-        for(int J = 0; J < I; ++J)
+    };
+    T ra[PEPT], rb[PEPT];
+    issue(ra);
+    issue(rb);
+
+    int cI = 0, cJ = 0; // tile being consumed
+    typename Mfma<T>::acc_t acc0{0, 0, 0, 0}, acc1{0, 0, 0, 0};
+
+    // consume the tile held in `reg`; true once the last block row is solved
+    auto step = [&](T(&reg)[PEPT]) -> bool {
+        if(cJ < cI)
         {
+            // ---- c_I accumulation: L_IJ y_J on MFMA (two chains) ----
             __syncthreads(); // previous readers of Ls are done
-            store_tile();
+            store_tile(reg);
             __syncthreads();
-            load_tile(I, J + 1); // next: (I, J+1), the diagonal block when J+1 == I
+            issue(reg);
 #pragma unroll
             for(int k4 = 0; k4 < PNB / 4; k4 += 2)
             {
                 const int kk0 = 4 * k4 + (lane >> 4);
                 const int kk1 = kk0 + 4;
                 const T   a0  = Ls[(16 * w + (lane & 15)) * PLDA + kk0];
-                const T   y0  = Ys[(J * PNB + kk0) * PLDY + (lane & 15)];
+                const T   y0  = Ys[(cJ * PNB + kk0) * PLDY + (lane & 15)];
                 const T   a1  = Ls[(16 * w + (lane & 15)) * PLDA + kk1];
-                const T   y1  = Ys[(J * PNB + kk1) * PLDY + (lane & 15)];
+                const T   y1  = Ys[(cJ * PNB + kk1) * PLDY + (lane & 15)];
                 acc0          = Mfma<T>::run(a0, y0, acc0);
                 acc1          = Mfma<T>::run(a1, y1, acc1);
             }
+            ++cJ;
+            return false;
         }
+        // diagonal block (I, I): -L below the diagonal, reciprocal diagonal.
+        // The block row's right-hand sides are loaded first: only the next
+        // tile's loads follow them, so waiting for them drains no prefetch
+        T bv[4];
+        load_rhs(bv, cI);
         __syncthreads();
-        // diagonal block (I, I): -L below the diagonal, reciprocal diagonal
-        store_diag();
+        store_diag(reg, cI);
 #pragma unroll
         for(int r = 0; r < 4; ++r)
             Cs[(16 * w + Mfma<T>::row(lane, r)) * PNC + (lane & 15)] = acc0[r] + acc1[r];
@@ -254,16 +276,14 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
         if(tid < PNB)
             rdiag[tid] = T(1) / rdiag[tid];
         __syncthreads();
-        // prefetch the first tile of the next block during the substitution
-        if(I + 1 < nblk)
-            load_tile(I + 1, 0);
+        issue(reg);
 
         // ---- y_I = L_II^{-1} c_I ----
         T c[4], rd[4];
 #pragma unroll
         for(int r = 0; r < 4; ++r)
         {
-            const int  gi = I * PNB + g + 16 * r;
+            const int  gi = cI * PNB + g + 16 * r;
             const bool ok = vok && gi < K;
             c[r]          = ok ? scale * bv[r] - Cs[(g + 16 * r) * PNC + vl] : T(0);
             rd[r]         = rdiag[g + 16 * r];
@@ -277,11 +297,11 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
             asm volatile("" : "+v"(c[r]));
             asm volatile("" : "+v"(rd[r]));
         }
-        fused_solve<T>(c, rd, Ls, g, min(PNB, K - I * PNB));
+        fused_solve<T>(c, rd, Ls, g, min(PNB, K - cI * PNB));
 #pragma unroll
         for(int r = 0; r < 4; ++r)
         {
-            const int gi = I * PNB + g + 16 * r;
+            const int gi = cI * PNB + g + 16 * r;
             const T   y  = c[r] * rd[r];
             if(gi < K)
             {
@@ -290,6 +310,20 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
                     gstore(Bp + boff(gi, vec), y);
             }
         }
+        ++cI;
+        cJ = 0;
+        if(cI >= nblk)
+            return true;
+        acc0 = typename Mfma<T>::acc_t{0, 0, 0, 0};
+        acc1 = typename Mfma<T>::acc_t{0, 0, 0, 0};
+        return false;
+    };
+    for(;;)
+    {
+        if(step(ra))
+            break;
+        if(step(rb))
+            break;
     }
     __syncthreads(); // LDS reuse by the caller's next panel
 }

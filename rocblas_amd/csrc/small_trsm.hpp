@@ -385,21 +385,6 @@ struct DppSolve
     }
 };
 
-// wave-uniform copy of a value every lane holds
-template <typename P>
-__device__ __forceinline__ P* uniform_ptr(P* p)
-{
-    const uint64_t v  = reinterpret_cast<uint64_t>(p);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
-    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
-    return reinterpret_cast<P*>((uint64_t(hi) << 32) | lo);
-}
-
-__device__ __forceinline__ int uniform_int(int v)
-{
-    return __builtin_amdgcn_readfirstlane(v);
-}
-
 template <typename T, int KMAX, bool UPPER, bool TA, bool COLS, bool FULL>
 __global__ void __launch_bounds__(WAVE) small_trsm_dpp_kernel(const SmallArgs<T> a)
 {

@@ -212,17 +212,20 @@ __global__ void __launch_bounds__(panel::PNT)
         nv = __builtin_amdgcn_readfirstlane(nv);
         const int G      = vb_groups(a, k, nv);
         const int panels = (nv + panel::PNC - 1) / panel::PNC;
-        const T*  Ap     = a.A[inst];
-        T*        Bp     = a.B[inst];
-        const T   alpha  = vb_alpha(a, inst);
-        const int flags  = vb_flags(a, inst);
+        // per-instance values read from device arrays are uniform too
+        const T*  Ap     = uniform_ptr(a.A[inst]);
+        T*        Bp     = uniform_ptr(a.B[inst]);
+        const T   alpha  = uniform_val(vb_alpha(a, inst));
+        const int flags  = uniform_int(vb_flags(a, inst));
+        const int lda    = uniform_int(a.lda[inst]);
+        const int ldb    = uniform_int(a.ldb[inst]);
         for(int p = g; p < panels; p += G)
         {
             if(alpha == T(0))
-                panel::panel_zero<T>(Bp, k, nv, a.ldb[inst], flags, p * panel::PNC);
+                panel::panel_zero<T>(Bp, k, nv, ldb, flags, p * panel::PNC);
             else
-                panel::panel_solve<T>(Ap, Bp, k, nv, a.lda[inst], a.ldb[inst], flags, alpha,
-                                      p * panel::PNC, vb_smem);
+                panel::panel_solve<T>(Ap, Bp, k, nv, lda, ldb, flags, alpha, p * panel::PNC,
+                                      vb_smem);
         }
     }
 }
