@@ -219,8 +219,13 @@ __global__ void __launch_bounds__(PNT)
     for(int r = 0; r < 4; ++r)
         rd[r] = T(1) / dg[g + 16 * r];
 
-    // ---- c_I accumulation: sum_{J<I} L_IJ y_J on MFMA (two chains) ----
-    typename Mfma<T>::acc_t acc0{0, 0, 0, 0}, acc1{0, 0, 0, 0};
+    // ---- c_I accumulation: sum_{J<I} L_IJ y_J on MFMA ----
+    typename Mfma<T>::acc_t acc[4];
+#pragma unroll
+    for(int a = 0; a < 4; ++a)
+        acc[a] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+    const T* lrow = Ls + (16 * w + (lane & 15)) * PLDA + (lane >> 4);
+    const T* ycol = Yj + (lane >> 4) * PLDY + (lane & 15);
     if(I > 0)
         load_tile(0);
     for(int J = 0; J < I; ++J)
@@ -242,22 +247,11 @@ __global__ void __launch_bounds__(PNT)
         __syncthreads();
         if(J + 1 < I)
             load_tile(J + 1);
-#pragma unroll
-        for(int k4 = 0; k4 < PNB / 4; k4 += 2)
-        {
-            const int kk0 = 4 * k4 + (lane >> 4);
-            const int kk1 = kk0 + 4;
-            const T   a0  = Ls[(16 * w + (lane & 15)) * PLDA + kk0];
-            const T   y0  = Yj[kk0 * PLDY + (lane & 15)];
-            const T   a1  = Ls[(16 * w + (lane & 15)) * PLDA + kk1];
-            const T   y1  = Yj[kk1 * PLDY + (lane & 15)];
-            acc0          = Mfma<T>::run(a0, y0, acc0);
-            acc1          = Mfma<T>::run(a1, y1, acc1);
-        }
+        mfma_tile<T>(acc, lrow, ycol);
     }
 #pragma unroll
     for(int r = 0; r < 4; ++r)
-        Cs[(16 * w + Mfma<T>::row(lane, r)) * PNC + (lane & 15)] = acc0[r] + acc1[r];
+        Cs[(16 * w + Mfma<T>::row(lane, r)) * PNC + (lane & 15)] = fold_acc<T>(acc, r);
     __syncthreads();
 
     // ---- y_I = L_II^{-1} c_I ----

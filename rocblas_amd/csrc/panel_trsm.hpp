@@ -37,11 +37,13 @@ constexpr int PEPT  = PNB * PNB / PNT; // tile elements per thread
 // Fused substitution: Ln holds -L of the diagonal block with
 // its diagonal and upper part zero, rd[r] = 1 / L_ii of this lane's rows
 // i = g + 16 r.  Step J: the owner lane (g == J % 16) holds
-// x_J = c_J * rd_J in register J / 16, and each update is one
+// x_J = c_J * rd_J in register q = J / 16, and each update is one
 // v_fmac_{f64,f32}_dpp broadcasting x_J from that lane:
 // c[r] += x_J * (-L[g + 16 r][J]).  Rows <= J are unchanged (their entries of
 // Ln are zero), so after the last step row i still holds c_i and
 // y_i = c_i * rd_i, which is x_i exactly as the step that owned row i computed it.
+// Registers r < q hold rows above J only, which are final: step J updates
+// (and loads the column entries of) registers q..3 alone.
 //
 // The inline-assembly updates are scheduling barriers, so the LDS reads of
 // column J are issued by hand FUSED_AHEAD steps early, into a ring of
@@ -53,7 +55,7 @@ template <typename T, int J>
 __device__ __forceinline__ void fused_load(T (&l)[4], const T* Ln, int g)
 {
 #pragma unroll
-    for(int r = 0; r < 4; ++r)
+    for(int r = J / 16; r < 4; ++r)
         l[r] = Ln[(g + 16 * r) * PLDA + J];
 }
 
@@ -77,10 +79,13 @@ struct FusedSolve
             T(&l)[4]         = lb[J % FUSED_RING];
             // x was written just before the step's first update; the solve's
             // very first update also covers a possible EXEC write
-            fmac_bcast<jl, J == 0 ? 5 : 2>(c[0], x, l[0]);
-            fmac_bcast<jl, 0>(c[1], x, l[1]);
-            fmac_bcast<jl, 0>(c[2], x, l[2]);
-            fmac_bcast<jl, 0>(c[3], x, l[3]);
+            fmac_bcast<jl, J == 0 ? 5 : 2>(c[q], x, l[q]);
+            if constexpr(q < 1)
+                fmac_bcast<jl, 0>(c[1], x, l[1]);
+            if constexpr(q < 2)
+                fmac_bcast<jl, 0>(c[2], x, l[2]);
+            if constexpr(q < 3)
+                fmac_bcast<jl, 0>(c[3], x, l[3]);
             FusedSolve<T, J + 1>::run(c, rd, Ln, g, nrows, lb);
         }
     }
@@ -98,6 +103,34 @@ __device__ __forceinline__ void fused_solve(T (&c)[4], const T (&rd)[4], const T
     fused_load<T, 3>(lb[3], Ln, g);
     static_assert(FUSED_AHEAD == 4, "prologue loads FUSED_AHEAD columns");
     FusedSolve<T, 0>::run(c, rd, Ln, g, nrows, lb);
+}
+
+// One 64 x 64 tile's update of a block's accumulators, sum_k L[row][k] y[k][v]
+// on MFMA: lrow = the tile in LDS at (16 w + lane % 16, lane / 16), ycol = the
+// solved block in LDS at (lane / 16, lane % 16) with row stride PLDY.  Every
+// fragment is read before the first product; the products alternate over four
+// accumulator chains (fold_acc sums them).  The chained and panel solves both
+// accumulate through this function, so they round identically.
+template <typename T>
+__device__ __forceinline__ void mfma_tile(typename Mfma<T>::acc_t (&acc)[4], const T* lrow,
+                                          const T* ycol)
+{
+    T af[PNB / 4], yf[PNB / 4];
+#pragma unroll
+    for(int k4 = 0; k4 < PNB / 4; ++k4)
+    {
+        af[k4] = lrow[4 * k4];
+        yf[k4] = ycol[4 * k4 * PLDY];
+    }
+#pragma unroll
+    for(int k4 = 0; k4 < PNB / 4; ++k4)
+        acc[k4 % 4] = Mfma<T>::run(af[k4], yf[k4], acc[k4 % 4]);
+}
+
+template <typename T>
+__device__ __forceinline__ T fold_acc(const typename Mfma<T>::acc_t (&acc)[4], int r)
+{
+    return (acc[0][r] + acc[1][r]) + (acc[2][r] + acc[3][r]);
 }
 
 // LDS bytes of one workgroup solving triangles of size up to K
@@ -236,29 +269,23 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
     issue(rb);
 
     int cI = 0, cJ = 0; // tile being consumed
-    typename Mfma<T>::acc_t acc0{0, 0, 0, 0}, acc1{0, 0, 0, 0};
+    typename Mfma<T>::acc_t acc[4];
+#pragma unroll
+    for(int a = 0; a < 4; ++a)
+        acc[a] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+    const T* lrow = Ls + (16 * w + (lane & 15)) * PLDA + (lane >> 4);
+    const T* ycol = Ys + (lane >> 4) * PLDY + (lane & 15);
 
     // consume the tile held in `reg`; true once the last block row is solved
     auto step = [&](T(&reg)[PEPT]) -> bool {
         if(cJ < cI)
         {
-            // ---- c_I accumulation: L_IJ y_J on MFMA (two chains) ----
+            // ---- c_I accumulation: L_IJ y_J on MFMA ----
             __syncthreads(); // previous readers of Ls are done
             store_tile(reg);
             __syncthreads();
             issue(reg);
-#pragma unroll
-            for(int k4 = 0; k4 < PNB / 4; k4 += 2)
-            {
-                const int kk0 = 4 * k4 + (lane >> 4);
-                const int kk1 = kk0 + 4;
-                const T   a0  = Ls[(16 * w + (lane & 15)) * PLDA + kk0];
-                const T   y0  = Ys[(cJ * PNB + kk0) * PLDY + (lane & 15)];
-                const T   a1  = Ls[(16 * w + (lane & 15)) * PLDA + kk1];
-                const T   y1  = Ys[(cJ * PNB + kk1) * PLDY + (lane & 15)];
-                acc0          = Mfma<T>::run(a0, y0, acc0);
-                acc1          = Mfma<T>::run(a1, y1, acc1);
-            }
+            mfma_tile<T>(acc, lrow, ycol + cJ * (PNB * PLDY));
             ++cJ;
             return false;
         }
@@ -271,7 +298,7 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
         store_diag(reg, cI);
 #pragma unroll
         for(int r = 0; r < 4; ++r)
-            Cs[(16 * w + Mfma<T>::row(lane, r)) * PNC + (lane & 15)] = acc0[r] + acc1[r];
+            Cs[(16 * w + Mfma<T>::row(lane, r)) * PNC + (lane & 15)] = fold_acc<T>(acc, r);
         __syncthreads();
         if(tid < PNB)
             rdiag[tid] = T(1) / rdiag[tid];
@@ -314,8 +341,9 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
         cJ = 0;
         if(cI >= nblk)
             return true;
-        acc0 = typename Mfma<T>::acc_t{0, 0, 0, 0};
-        acc1 = typename Mfma<T>::acc_t{0, 0, 0, 0};
+#pragma unroll
+        for(int a = 0; a < 4; ++a)
+            acc[a] = typename Mfma<T>::acc_t{0, 0, 0, 0};
         return false;
     };
     for(;;)
