@@ -220,9 +220,9 @@ __global__ void __launch_bounds__(PNT)
         rd[r] = T(1) / dg[g + 16 * r];
 
     // ---- c_I accumulation: sum_{J<I} L_IJ y_J on MFMA ----
-    typename Mfma<T>::acc_t acc[4];
+    typename Mfma<T>::acc_t acc[MCH];
 #pragma unroll
-    for(int a = 0; a < 4; ++a)
+    for(int a = 0; a < MCH; ++a)
         acc[a] = typename Mfma<T>::acc_t{0, 0, 0, 0};
     const T* lrow = Ls + (16 * w + (lane & 15)) * PLDA + (lane >> 4);
     const T* ycol = Yj + (lane >> 4) * PLDY + (lane & 15);

@@ -9,7 +9,8 @@ namespace rbamd
 namespace
 {
 template <typename T>
-__global__ void __launch_bounds__(panel::PNT) panel_trsm_kernel(const SmallArgs<T> a)
+__global__ void __launch_bounds__(panel::PNT) __attribute__((amdgpu_waves_per_eu(2)))
+    panel_trsm_kernel(const SmallArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char panel_smem[];
     const T   alpha = a.alpha_dev ? *a.alpha_dev : a.alpha_host;

@@ -8,7 +8,7 @@
 // M = op(A)^T; SF_REV reverses the index).  The triangle is walked in 64-row
 // blocks I, left-looking:
 //
-//   c_I = scale * b_I - sum_{J<I} L_IJ y_J    (FP64/FP32 MFMA, y_J from LDS)
+//   c_I = scale * b_I - sum_{J<I} L_IJ y_J    (FP64/FP32 MFMA)
 //   y_I = L_II^{-1} c_I                       (substitution, DPP multiply-adds)
 //
 // Four wavefronts: in the update each owns 16 rows of the block (one 16x16
@@ -18,7 +18,9 @@
 // v_fmac_dpp that broadcasts it from lane j%16 (row_newbcast).  Tiles of L
 // are staged through LDS with coalesced loads (either storage order of A),
 // the next two tiles' loads in flight during the current tile's MFMAs or
-// substitution.
+// substitution.  The solved rows y_J are not kept: each block row reads them
+// back from B (where the substitution stored them), one tile ahead, so the
+// workgroup's LDS does not grow with K and two workgroups share a CU.
 #pragma once
 
 #include "device_common.hpp"
@@ -33,6 +35,7 @@ constexpr int PLDA  = PNB + 1;  // LDS row stride of a staged tile
 constexpr int PLDY  = PNC + 1;  // LDS row stride of the solved panel
 constexpr int PNT   = 256;      // threads
 constexpr int PEPT  = PNB * PNB / PNT; // tile elements per thread
+constexpr int MCH   = 2;        // MFMA accumulator chains of a tile update
 
 // Fused substitution: Ln holds -L of the diagonal block with
 // its diagonal and upper part zero, rd[r] = 1 / L_ii of this lane's rows
@@ -107,37 +110,46 @@ __device__ __forceinline__ void fused_solve(T (&c)[4], const T (&rd)[4], const T
 
 // One 64 x 64 tile's update of a block's accumulators, sum_k L[row][k] y[k][v]
 // on MFMA: lrow = the tile in LDS at (16 w + lane % 16, lane / 16), ycol = the
-// solved block in LDS at (lane / 16, lane % 16) with row stride PLDY.  Every
-// fragment is read before the first product; the products alternate over four
-// accumulator chains (fold_acc sums them).  The chained and panel solves both
+// solved block in LDS at (lane / 16, lane % 16) with row stride PLDY.  Each
+// half of the fragments is read before its first product; the products
+// alternate over MCH accumulator chains (fold_acc sums them).  The chained and panel solves both
 // accumulate through this function, so they round identically.
 template <typename T>
-__device__ __forceinline__ void mfma_tile(typename Mfma<T>::acc_t (&acc)[4], const T* lrow,
+__device__ __forceinline__ void mfma_tile(typename Mfma<T>::acc_t (&acc)[MCH], const T* lrow,
                                           const T* ycol)
 {
-    T af[PNB / 4], yf[PNB / 4];
+    constexpr int HALF = PNB / 8;
 #pragma unroll
-    for(int k4 = 0; k4 < PNB / 4; ++k4)
+    for(int h = 0; h < 2; ++h)
     {
-        af[k4] = lrow[4 * k4];
-        yf[k4] = ycol[4 * k4 * PLDY];
-    }
+        T af[HALF], yf[HALF];
 #pragma unroll
-    for(int k4 = 0; k4 < PNB / 4; ++k4)
-        acc[k4 % 4] = Mfma<T>::run(af[k4], yf[k4], acc[k4 % 4]);
+        for(int k = 0; k < HALF; ++k)
+        {
+            af[k] = lrow[4 * (h * HALF + k)];
+            yf[k] = ycol[4 * (h * HALF + k) * PLDY];
+        }
+#pragma unroll
+        for(int k = 0; k < HALF; ++k)
+            acc[k % MCH] = Mfma<T>::run(af[k], yf[k], acc[k % MCH]);
+    }
 }
 
 template <typename T>
-__device__ __forceinline__ T fold_acc(const typename Mfma<T>::acc_t (&acc)[4], int r)
+__device__ __forceinline__ T fold_acc(const typename Mfma<T>::acc_t (&acc)[MCH], int r)
 {
-    return (acc[0][r] + acc[1][r]) + (acc[2][r] + acc[3][r]);
+    static_assert(MCH == 2 || MCH == 4, "two or four chains");
+    if constexpr(MCH == 2)
+        return acc[0][r] + acc[1][r];
+    else
+        return (acc[0][r] + acc[1][r]) + (acc[2][r] + acc[3][r]);
 }
 
-// LDS bytes of one workgroup solving triangles of size up to K
+// LDS bytes of one workgroup (the same for every triangle size)
 template <typename T>
-__host__ __device__ constexpr size_t lds_bytes(int K)
+__host__ __device__ constexpr size_t lds_bytes(int /*K*/)
 {
-    return sizeof(T) * (size_t(PNB) * PLDA + size_t(PNB) * PNC + PNB + size_t(K) * PLDY);
+    return sizeof(T) * (size_t(PNB) * PLDA + size_t(PNB) * PNC + PNB + size_t(PNB) * PLDY);
 }
 
 // B := 0 for the panel's vectors (alpha == 0; A is not referenced)
@@ -163,7 +175,7 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
     T* Ls    = reinterpret_cast<T*>(smem); // PNB x PLDA staged tile
     T* Cs    = Ls + PNB * PLDA;            // PNB x PNC accumulated update
     T* rdiag = Cs + PNB * PNC;             // PNB reciprocal diagonal
-    T* Ys    = rdiag + PNB;                // K x PLDY solved panel
+    T* Yt    = rdiag + PNB;                // PNB x PLDY solved block y_J of the tile
 
     const int  tid  = threadIdx.x;
     const int  lane = tid & (WAVE - 1);
@@ -244,6 +256,35 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
             bv[r] = gload(Bp + boff(min(I * PNB + g + 16 * r, K - 1), vclamp));
     };
 
+    // y_J of block row J for the panel's vectors, read back from B: 64 rows x
+    // 16 vectors, four per thread, consecutive lanes along B's contiguous
+    // dimension.  Rows past K and vectors past nvec read row K - 1 / vector
+    // nvec - 1 (in bounds; such values never reach a stored result).
+    auto y_rv = [&](int s, int& r, int& v) {
+        const int e = tid + PNT * s;
+        r           = cols ? e % PNB : e / PNC;
+        v           = cols ? e / PNB : e % PNC;
+    };
+    auto load_y = [&](T(&yv)[4], int J) {
+#pragma unroll
+        for(int s = 0; s < 4; ++s)
+        {
+            int r, v;
+            y_rv(s, r, v);
+            yv[s] = gload(Bp + boff(min(J * PNB + r, K - 1), min(v0 + v, nvec - 1)));
+        }
+    };
+    auto store_y = [&](const T(&yv)[4]) {
+#pragma unroll
+        for(int s = 0; s < 4; ++s)
+        {
+            int r, v;
+            y_rv(s, r, v);
+            Yt[r * PLDY + v] = yv[s];
+        }
+    };
+    static_assert(PNB * PNC == 4 * PNT, "four y values per thread");
+
     // The tiles are consumed in the order (0,0), (1,0), (1,1), (2,0), ...,
     // (I, I) being the diagonal block that ends block row I.  Two tiles are
     // in flight at any time (register buffers ra and rb, used alternately):
@@ -264,28 +305,32 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
             iJ = 0;
         }
     };
-    T ra[PEPT], rb[PEPT];
+    T ra[PEPT];
     issue(ra);
-    issue(rb);
+    T yv[4]; // y of the next off-diagonal tile (loaded one step ahead)
 
     int cI = 0, cJ = 0; // tile being consumed
-    typename Mfma<T>::acc_t acc[4];
+    typename Mfma<T>::acc_t acc[MCH];
 #pragma unroll
-    for(int a = 0; a < 4; ++a)
+    for(int a = 0; a < MCH; ++a)
         acc[a] = typename Mfma<T>::acc_t{0, 0, 0, 0};
     const T* lrow = Ls + (16 * w + (lane & 15)) * PLDA + (lane >> 4);
-    const T* ycol = Ys + (lane >> 4) * PLDY + (lane & 15);
+    const T* ycol = Yt + (lane >> 4) * PLDY + (lane & 15);
 
     // consume the tile held in `reg`; true once the last block row is solved
     auto step = [&](T(&reg)[PEPT]) -> bool {
         if(cJ < cI)
         {
             // ---- c_I accumulation: L_IJ y_J on MFMA ----
-            __syncthreads(); // previous readers of Ls are done
+            __syncthreads(); // previous readers of Ls and Yt are done
             store_tile(reg);
+            store_y(yv);
             __syncthreads();
             issue(reg);
-            mfma_tile<T>(acc, lrow, ycol + cJ * (PNB * PLDY));
+            // y of the next tile: y_{J+1}, or (the diagonal tile is next) a
+            // load nobody reads, so every step issues the same loads
+            load_y(yv, cJ + 1 < cI ? cJ + 1 : 0);
+            mfma_tile<T>(acc, lrow, ycol);
             ++cJ;
             return false;
         }
@@ -329,29 +374,23 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
         for(int r = 0; r < 4; ++r)
         {
             const int gi = cI * PNB + g + 16 * r;
-            const T   y  = c[r] * rd[r];
-            if(gi < K)
-            {
-                Ys[gi * PLDY + vl] = y;
-                if(vok)
-                    gstore(Bp + boff(gi, vec), y);
-            }
+            if(gi < K && vok)
+                gstore(Bp + boff(gi, vec), c[r] * rd[r]);
         }
         ++cI;
         cJ = 0;
         if(cI >= nblk)
             return true;
+        // every wavefront's rows of y_I are in B before any is read back
+        __syncthreads();
+        load_y(yv, 0); // the next tile is (I + 1, 0)
 #pragma unroll
-        for(int a = 0; a < 4; ++a)
+        for(int a = 0; a < MCH; ++a)
             acc[a] = typename Mfma<T>::acc_t{0, 0, 0, 0};
         return false;
     };
-    for(;;)
+    while(!step(ra))
     {
-        if(step(ra))
-            break;
-        if(step(rb))
-            break;
     }
     __syncthreads(); // LDS reuse by the caller's next panel
 }

@@ -75,7 +75,7 @@ struct VbTuning
     int nb16_max_k       = kVbNb16MaxK;
     int waves_per_cu     = kVbWavesPerCu;
     int groups[kVbPanelBuckets];
-    int panel_wgs_per_cu = 1;
+    int panel_wgs_per_cu = 2;
     VbTuning()
     {
         for(int b = 0; b < kVbPanelBuckets; ++b)
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(1024) vb_plan_kernel(const VbArgs<T> a)
 
 // Panel kernel: persistent workgroups over the panel work items.
 template <typename T>
-__global__ void __launch_bounds__(panel::PNT)
+__global__ void __launch_bounds__(panel::PNT) __attribute__((amdgpu_waves_per_eu(2)))
     vb_panel_kernel(const VbArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char vb_smem[];
