@@ -225,12 +225,22 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
     // An off-diagonal tile (J < I) is dense and its columns are all below K;
     // rows past K (last block row) carry row K - 1's values into MFMA result
     // rows that are discarded, so it is stored without a mask.
-    const int lds0 = tA ? wu * PLDA + lane : lane * PLDA + wu; // element 0 of this thread
-    const int ldss = tA ? 4 * PLDA : 4;                        // step between elements
-    auto      store_tile = [&](const T(&reg)[PEPT]) {
+    // (two loops, so each element's LDS address is an immediate offset from
+    // one base register)
+    T* const lds0       = Ls + (tA ? wu * PLDA + lane : lane * PLDA + wu); // element 0
+    auto     store_tile = [&](const T(&reg)[PEPT]) {
+        if(tA)
+        {
 #pragma unroll
-        for(int s = 0; s < PEPT; ++s)
-            Ls[lds0 + s * ldss] = reg[s];
+            for(int s = 0; s < PEPT; ++s)
+                lds0[s * 4 * PLDA] = reg[s];
+        }
+        else
+        {
+#pragma unroll
+            for(int s = 0; s < PEPT; ++s)
+                lds0[s * 4] = reg[s];
+        }
     };
     // the diagonal block: -L strictly below the diagonal (the fused
     // substitution's operand), the diagonal into rdiag

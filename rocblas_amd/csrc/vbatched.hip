@@ -191,6 +191,8 @@ __global__ void __launch_bounds__(panel::PNT) __attribute__((amdgpu_waves_per_eu
     extern __shared__ __align__(16) unsigned char vb_smem[];
     __shared__ int item_s;
     const int      total = a.counter[2];
+    if(int(blockIdx.x) >= total)
+        return; // more workgroups than work items: the rest take them all
     for(;;)
     {
         if(threadIdx.x == 0)
@@ -352,6 +354,10 @@ __global__ void __launch_bounds__(WAVE) vb_solve_kernel(const VbArgs<T> a)
     __shared__ int item_s;
 
     const int  total = a.counter[3];
+    // the first `total` wavefronts take every instance between them; the
+    // rest leave without touching the shared counter
+    if(int(blockIdx.x) >= total)
+        return;
     for(;;)
     {
         if(threadIdx.x == 0)
