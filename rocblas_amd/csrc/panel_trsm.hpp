@@ -17,10 +17,12 @@
 // and every update of row i by the solved value of row j is one
 // v_fmac_dpp that broadcasts it from lane j%16 (row_newbcast).  Tiles of L
 // are staged through LDS with coalesced loads (either storage order of A),
-// the next two tiles' loads in flight during the current tile's MFMAs or
+// the next tile's loads in flight during the current tile's MFMAs or
 // substitution.  The solved rows y_J are not kept: each block row reads them
 // back from B (where the substitution stored them), one tile ahead, so the
-// workgroup's LDS does not grow with K and two workgroups share a CU.
+// workgroup's LDS does not grow with K and two workgroups share a CU (which
+// hides each one's waits better than a deeper prefetch: the registers of a
+// second tile in flight would not fit two waves per SIMD).
 #pragma once
 
 #include "device_common.hpp"
@@ -296,11 +298,9 @@ __device__ void panel_solve(const T* __restrict__ Ap, T* Bp, int K, int nvec, in
     static_assert(PNB * PNC == 4 * PNT, "four y values per thread");
 
     // The tiles are consumed in the order (0,0), (1,0), (1,1), (2,0), ...,
-    // (I, I) being the diagonal block that ends block row I.  Two tiles are
-    // in flight at any time (register buffers ra and rb, used alternately):
-    // a buffer is refilled with the tile two places ahead as soon as its
-    // tile is in LDS, so each tile's load latency overlaps the two steps
-    // before it is needed.
+    // (I, I) being the diagonal block that ends block row I.  The register
+    // buffer ra is refilled with the next tile as soon as its tile is in
+    // LDS, so each tile's load latency overlaps the step before it is needed.
     // Every step issues exactly one tile's loads (past the last tile, the
     // last tile again, never consumed), so each path through a step leaves
     // the same number of loads in flight and the compiler's wait counts
