@@ -218,7 +218,8 @@ constexpr int slot_elems(int kmax)
     return (kmax * (kmax + 1) / 2) | 1;
 }
 
-// Single-launch panel solve for SMALL_K < k <= kPanelMaxK (panel_trsm.hip):
+// Single-launch panel solve for k <= kPanelMaxK (panel_trsm.hip; below 64 only
+// where the driver prefers it to the small kernels):
 // one workgroup per 16 vectors, same SmallArgs (K, nvec, lda, ldb, flags,
 // alpha); `units` is ignored (the batch count is passed separately).
 constexpr int kPanelMaxK = 512;

@@ -232,6 +232,24 @@ static rocblas_status launch_small(rocblas_handle h, const SmallArgs<T>& args_in
 // Driver
 // ---------------------------------------------------------------------------
 constexpr int SMALL_K = 64; // largest k solved by the small kernel / leaf size
+
+// A solve with k <= SMALL_K takes the small kernels (triangle in registers or
+// in one wavefront's LDS) except where the panel solve is faster: 32 < k with
+// at most 16 vectors, or 48 < k with at most 32.  Measured on MI355X, side L,
+// strided batch of 20000 (small / panel):
+//   k = 64, n = 16: 1000 / 416 us;  k = 48, n = 16: 380 / 377 us;
+//   k = 64, n = 64: 1076 / 1097 us; k = 48, n = 64: 548 / 1027 us;
+//   one instance, k = 64, n = 16: 38 / 17 us.
+// The choice depends on k and the vector count only, so a batched solve and
+// the single solves of its instances take the same kernels (and bits).
+inline bool small_kernel_preferred(int K, int nvec)
+{
+    if(K <= 32)
+        return true;
+    if(nvec <= 16)
+        return false;
+    return !(K > 48 && nvec <= 32);
+}
 // leaves of the recursion for right-hand sides of at least kWidePanels panels
 constexpr int kWideLeafK  = 256;
 constexpr int kWidePanels = 64;
@@ -387,7 +405,7 @@ rocblas_status trsm_launch(rocblas_handle h, const TrsmProblem<T>& p)
     const bool  lowerM = (!upperA) != tA;
     const int   K      = sideL ? p.m : p.n;
 
-    if(K <= SMALL_K)
+    if(K <= SMALL_K && small_kernel_preferred(K, sideL ? p.n : p.m))
     {
         SmallArgs<T> s{};
         s.A          = p.A;
