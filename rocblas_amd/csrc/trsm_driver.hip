@@ -234,21 +234,18 @@ static rocblas_status launch_small(rocblas_handle h, const SmallArgs<T>& args_in
 constexpr int SMALL_K = 64; // largest k solved by the small kernel / leaf size
 
 // A solve with k <= SMALL_K takes the small kernels (triangle in registers or
-// in one wavefront's LDS) except where the panel solve is faster: 32 < k with
-// at most 16 vectors, or 48 < k with at most 32.  Measured on MI355X, side L,
-// strided batch of 20000 (small / panel):
+// in one wavefront's LDS) except where the panel solve is as fast or faster:
+// 48 < k, or 32 < k with at most 16 vectors.  Measured on MI355X, side L,
+// strided batch of 20000 (small kernels / panel kernel behind the variable-
+// size batch's planning launch):
 //   k = 64, n = 16: 1000 / 416 us;  k = 48, n = 16: 380 / 377 us;
 //   k = 64, n = 64: 1076 / 1097 us; k = 48, n = 64: 548 / 1027 us;
-//   one instance, k = 64, n = 16: 38 / 17 us.
+//   one instance, k = 64, n = 16: 38 / 17 us; k = 64, n = 64: 39 / 31 us.
 // The choice depends on k and the vector count only, so a batched solve and
 // the single solves of its instances take the same kernels (and bits).
 inline bool small_kernel_preferred(int K, int nvec)
 {
-    if(K <= 32)
-        return true;
-    if(nvec <= 16)
-        return false;
-    return !(K > 48 && nvec <= 32);
+    return K <= 32 || (K <= 48 && nvec > 16);
 }
 // leaves of the recursion for right-hand sides of at least kWidePanels panels
 constexpr int kWideLeafK  = 256;
