@@ -71,8 +71,8 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_dtrsm_vbatched(rocblas_handle     
  * reference; for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
  * arguments, checks and results (bit for bit) as rocblas_amd_dtrsm_vbatched,
- * but the planning scratch (64 + 36 * batch_count bytes) lives in a
- * library-owned device array of 2,359,360 bytes per GPU, held until the
+ * but the planning scratch (64 + 32 * batch_count bytes) lives in a
+ * library-owned device array of 2,097,216 bytes per GPU, held until the
  * process ends, instead of the handle workspace: batch_count may be at most
  * 65536 (else rocblas_status_invalid_size), and a size query answers
  * rocblas_status_size_unchanged.  The first call on a device looks up the
@@ -302,20 +302,15 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_server_result(rocblas_amd_server s
                                                             void*              X,
                                                             rocblas_int        ldx);
 
-/* Overrides the tuning of the _vbatched solves for this process: for k > 512
- * instances (one wavefront each), k <= nb16_max_k uses 16-row blocks (else
- * 32-row blocks) and the launch keeps waves_per_cu wavefronts per compute
- * unit (1..32); instances with k <= 512 are split into up to panel_groups
- * work items (1..8) of the panel solve.  0 restores the built-in value of
- * any of them.  Used by the autotuner (rocblas_amd/autotune_vbatched.py); not
- * thread-safe against concurrent _vbatched calls.  No counterpart in the
- * reference. */
-ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb16_max_k,
-                                                                  rocblas_int waves_per_cu,
-                                                                  rocblas_int panel_groups);
+/* Overrides the tuning of the _vbatched solves for this process: every
+ * instance is split into up to panel_groups work items (1..8) of the panel
+ * solve; 0 restores the built-in table.  Used by the autotuner
+ * (rocblas_amd/autotune_vbatched.py); not thread-safe against concurrent
+ * _vbatched calls.  No counterpart in the reference. */
+ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int panel_groups);
 
 /* Per-size-bucket override of the same table: instances with k in bucket
- * (0: k <= 64, 1: <= 128, 2: <= 256, 3: <= 512) are split into panel_groups
+ * (0: k <= 64, 1: <= 128, 2: <= 256, 3: larger) are split into panel_groups
  * work items (1..8; 0 restores the built-in value).  rocblas_amd_vbatched_set_tuning
  * with panel_groups != 0 sets every bucket.  Used by the autotuner; not
  * thread-safe against concurrent _vbatched calls.  No counterpart in the

@@ -1,5 +1,5 @@
 // vbatched.hip -- variable-size batched triangular solve in one planning
-// launch plus two persistent launches, graph-capturable.
+// launch plus one persistent launch, graph-capturable.
 //
 //   rocblas_amd_{s,d}trsm_vbatched(handle, side, uplo, transA, diag,
 //                                  m[], n[], alpha, A[], lda[], B[], ldb[], batch)
@@ -9,25 +9,16 @@
 // counterpart: rocBLAS has no variable-size batched trsm, and its callers loop
 // over rocblas_dtrsm (one to thirty launches per instance).
 //
-// Plan kernel (1 workgroup): every valid instance with k <= 512 becomes up to
+// Plan kernel (1 workgroup): every valid instance becomes up to
 // kVbPanelGroups work items, each a strided share of the instance's 16-vector
 // panels; the items are counting-sorted by cost bucket
-// (floor(log2(k^2 * vectors))), largest first.  Instances with k > 512 go to a
-// second list.  Both work counters are reset with plain stores (no memset
-// node in a graph).
+// (floor(log2(k^2 * vectors))), largest first.  The work counter is reset
+// with a plain store (no memset node in a graph).
 //
-// Panel kernel (persistent, 256 threads, one workgroup per CU): each
+// Panel kernel (persistent, 256 threads, two workgroups per CU): each
 // workgroup takes the next item and solves its panels with the MFMA panel
-// solve of panel_trsm.hpp.
-//
-// Wave kernel (persistent, one wavefront per workgroup) for k > 512: each
-// wavefront takes the next instance and solves it on its own, 64
-// right-hand-side vectors at a time (lane = vector): blocked, left-looking
-// over NB-row blocks of the canonical triangle,
-//    c_I = alpha * b_I - sum_{J<I} L_IJ y_J ;  y_I = L_II^{-1} c_I
-// with each L block staged through LDS and read as LDS broadcasts.  NB is 16
-// or 32 by the size-bucket table in vbatched_tuning.hpp (written by
-// rocblas_amd/autotune_vbatched.py).
+// solve of panel_trsm.hpp, whose LDS does not depend on k, so every triangle
+// size takes this one path.
 // Results do not depend on the order in which workers take items.
 #include "panel_trsm.hpp"
 #include "vbatched.hpp"
@@ -60,10 +51,7 @@ struct VbArgs
     const int*         flags_arr;
     const T*           alpha_arr;
     int*               items;   // workspace: panel work items (inst * 8 + group, groups * batch)
-    int*               order;   // workspace: k > 512 instances (batch ints)
-    int*               counter; // workspace: [0] panel items taken, [1] large taken,
-                                //            [2] panel items, [3] large instances
-    int                nb16_max_k; // wave kernel: k <= this: 16-row blocks, else 32
+    int*               counter; // workspace: [0] items taken, [1] items
     int                groups[kVbPanelBuckets]; // work items per instance of the panel
                                                 // kernel (1..8), by size bucket of k
 };
@@ -72,8 +60,6 @@ struct VbArgs
 // overridden at run time (rocblas_amd_vbatched_set_tuning) by the autotuner.
 struct VbTuning
 {
-    int nb16_max_k       = kVbNb16MaxK;
-    int waves_per_cu     = kVbWavesPerCu;
     int groups[kVbPanelBuckets];
     int panel_wgs_per_cu = 2;
     VbTuning()
@@ -86,12 +72,12 @@ static VbTuning g_vb_tuning;
 
 constexpr int VB_BUCKETS = 64;
 
-// size bucket of the panel kernel's tuning table: k <= 64, <= 128, <= 256, <= 512
+// size bucket of the panel kernel's tuning table: k <= 64, <= 128, <= 256, larger
 __host__ __device__ constexpr int vb_panel_bucket(int k)
 {
     return k <= 64 ? 0 : k <= 128 ? 1 : k <= 256 ? 2 : 3;
 }
-static_assert(kVbPanelBuckets == 4, "four size buckets up to kPanelMaxK");
+static_assert(kVbPanelBuckets == 4, "four size buckets");
 
 __device__ __forceinline__ int vb_cost_bucket(long long k, long long nv)
 {
@@ -135,25 +121,17 @@ __global__ void __launch_bounds__(1024) vb_plan_kernel(const VbArgs<T> a)
 {
     __shared__ int hist[VB_BUCKETS];
     __shared__ int offs[VB_BUCKETS];
-    __shared__ int nlarge;
     for(int i = threadIdx.x; i < VB_BUCKETS; i += blockDim.x)
         hist[i] = 0;
-    if(threadIdx.x == 0)
-        nlarge = 0;
     __syncthreads();
     for(int i = threadIdx.x; i < a.batch; i += blockDim.x)
     {
         int k, nv;
         if(!vb_shape(a, i, k, nv))
             continue;
-        if(k <= kPanelMaxK)
-        {
-            const int G   = vb_groups(a, k, nv);
-            const int per = ((nv + panel::PNC - 1) / panel::PNC + G - 1) / G;
-            atomicAdd(&hist[vb_cost_bucket(k, min(nv, per * panel::PNC))], G);
-        }
-        else
-            a.order[atomicAdd(&nlarge, 1)] = i;
+        const int G   = vb_groups(a, k, nv);
+        const int per = ((nv + panel::PNC - 1) / panel::PNC + G - 1) / G;
+        atomicAdd(&hist[vb_cost_bucket(k, min(nv, per * panel::PNC))], G);
     }
     __syncthreads();
     if(threadIdx.x == 0)
@@ -165,15 +143,13 @@ __global__ void __launch_bounds__(1024) vb_plan_kernel(const VbArgs<T> a)
             t += hist[b];
         }
         a.counter[0] = 0;
-        a.counter[1] = 0;
-        a.counter[2] = t;
-        a.counter[3] = nlarge;
+        a.counter[1] = t;
     }
     __syncthreads();
     for(int i = threadIdx.x; i < a.batch; i += blockDim.x)
     {
         int k, nv;
-        if(!vb_shape(a, i, k, nv) || k > kPanelMaxK)
+        if(!vb_shape(a, i, k, nv))
             continue;
         const int G   = vb_groups(a, k, nv);
         const int per = ((nv + panel::PNC - 1) / panel::PNC + G - 1) / G;
@@ -190,7 +166,7 @@ __global__ void __launch_bounds__(panel::PNT) __attribute__((amdgpu_waves_per_eu
 {
     extern __shared__ __align__(16) unsigned char vb_smem[];
     __shared__ int item_s;
-    const int      total = a.counter[2];
+    const int      total = a.counter[1];
     if(int(blockIdx.x) >= total)
         return; // more workgroups than work items: the rest take them all
     for(;;)
@@ -232,177 +208,6 @@ __global__ void __launch_bounds__(panel::PNT) __attribute__((amdgpu_waves_per_eu
     }
 }
 
-// One instance, one chunk of up to 64 vectors, block size NB.
-template <typename T, int NB>
-__device__ void vb_solve_chunk(int flags, T* Ls, int K, int nvec, int c0, const T* Ap, int lda,
-                               T* Bp, int ldb, T alpha)
-{
-    constexpr int LDL  = NB + 1; // LDS row stride of a staged block
-    const int     lane = threadIdx.x;
-    const bool    tA   = flags & SF_TA;
-    const bool    rev  = flags & SF_REV;
-    const bool    unit = flags & SF_UNIT;
-    const bool    cols = flags & SF_COLS;
-    const int     vec  = c0 + lane;
-    const bool    vok  = vec < nvec;
-    const int     nblk = (K + NB - 1) / NB;
-
-    // element (canonical row r, vector vec) of B
-    auto bidx = [&](int r) -> int64_t {
-        const int real = rev ? K - 1 - r : r;
-        return cols ? int64_t(real) + int64_t(vec) * ldb : int64_t(vec) + int64_t(real) * ldb;
-    };
-    // stage canonical block L[I*NB.., J*NB..] (NB x NB, zero outside K) into Ls[i*LDL + j]
-    auto stage = [&](int I, int J, bool diagblk) {
-        for(int e = lane; e < NB * NB; e += WAVE)
-        {
-            // walk A's storage order so the loads are coalesced: A[p + q*lda]
-            // has consecutive p for consecutive e
-            const int sp = e % NB, sq = e / NB;
-            int       i, j; // canonical block-local coordinates
-            {
-                // canonical (gi, gj) <-> real M indices (mi, mj) <-> A (p, q)
-                // choose (i, j) such that p varies with sp
-                if(!tA)
-                {
-                    // p = real row mi, contiguous in canonical i (forward or reversed)
-                    i = sp;
-                    j = sq;
-                }
-                else
-                {
-                    // p = real mj
-                    i = sq;
-                    j = sp;
-                }
-            }
-            const int gi = I * NB + i, gj = J * NB + j;
-            T         v  = T(0);
-            if(gi < K && gj < K && (!diagblk || gi >= gj))
-            {
-                const int mi = rev ? K - 1 - gi : gi;
-                const int mj = rev ? K - 1 - gj : gj;
-                const int p  = tA ? mj : mi;
-                const int q  = tA ? mi : mj;
-                if(gi == gj)
-                    v = unit ? T(1) : T(1) / gload(Ap + (p + int64_t(q) * lda));
-                else
-                    v = gload(Ap + (p + int64_t(q) * lda));
-            }
-            else if(diagblk && i == j)
-                v = T(1);
-            Ls[i * LDL + j] = v;
-        }
-    };
-
-    for(int I = 0; I < nblk; ++I)
-    {
-        T c[NB];
-#pragma unroll
-        for(int r = 0; r < NB; ++r)
-        {
-            const int  gr = I * NB + r;
-            const bool ok = vok && gr < K;
-            c[r]          = ok ? alpha * gload(Bp + bidx(gr)) : T(0);
-        }
-        for(int J = 0; J < I; ++J)
-        {
-            stage(I, J, false);
-            T y[NB];
-#pragma unroll
-            for(int s = 0; s < NB; ++s)
-            {
-                const int gs = J * NB + s;
-                y[s]         = (vok && gs < K) ? gload(Bp + bidx(gs)) : T(0);
-            }
-            __syncthreads();
-#pragma unroll
-            for(int s = 0; s < NB; ++s)
-            {
-#pragma unroll
-                for(int r = 0; r < NB; ++r)
-                    c[r] = fma(-Ls[r * LDL + s], y[s], c[r]);
-            }
-            __syncthreads();
-        }
-        stage(I, I, true);
-        __syncthreads();
-#pragma unroll
-        for(int j = 0; j < NB; ++j)
-        {
-            const T x = c[j] * Ls[j * LDL + j];
-            c[j]      = x;
-#pragma unroll
-            for(int i = j + 1; i < NB; ++i)
-                c[i] = fma(-Ls[i * LDL + j], x, c[i]);
-        }
-#pragma unroll
-        for(int r = 0; r < NB; ++r)
-        {
-            const int gr = I * NB + r;
-            if(vok && gr < K)
-                gstore(Bp + bidx(gr), c[r]);
-        }
-        __syncthreads();
-    }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(WAVE) vb_solve_kernel(const VbArgs<T> a)
-{
-    __shared__ T Ls[32 * 33];
-    __shared__ int item_s;
-
-    const int  total = a.counter[3];
-    // the first `total` wavefronts take every instance between them; the
-    // rest leave without touching the shared counter
-    if(int(blockIdx.x) >= total)
-        return;
-    for(;;)
-    {
-        if(threadIdx.x == 0)
-            item_s = atomicAdd(&a.counter[1], 1);
-        __syncthreads();
-        const int item = item_s;
-        __syncthreads();
-        if(item >= total)
-            break;
-        const int inst  = a.order[item];
-        const int flags = vb_flags(a, inst);
-        const T   alpha = vb_alpha(a, inst);
-        const bool cols = flags & SF_COLS;
-        const int m    = a.m[inst];
-        const int n    = a.n[inst];
-        const int K    = cols ? m : n;
-        const int nvec = cols ? n : m;
-        const int lda = a.lda[inst];
-        const int ldb = a.ldb[inst];
-        // empty or malformed instances (lda < k, ldb < m) are left untouched
-        if(m <= 0 || n <= 0 || lda < K || ldb < m)
-            continue;
-        const T* Ap = a.A[inst];
-        T*       Bp = a.B[inst];
-        for(int c0 = 0; c0 < nvec; c0 += WAVE)
-        {
-            if(alpha == T(0))
-            {
-                // B := 0, A not referenced
-                const int vec = c0 + threadIdx.x;
-                if(vec < nvec)
-                    for(int r = 0; r < K; ++r)
-                        gstore(Bp + (cols ? int64_t(r) + int64_t(vec) * ldb
-                                          : int64_t(vec) + int64_t(r) * ldb),
-                               T(0));
-                continue;
-            }
-            if(K <= a.nb16_max_k)
-                vb_solve_chunk<T, 16>(flags, Ls, K, nvec, c0, Ap, lda, Bp, ldb, alpha);
-            else
-                vb_solve_chunk<T, 32>(flags, Ls, K, nvec, c0, Ap, lda, Bp, ldb, alpha);
-        }
-    }
-}
-
 template <typename T>
 rocblas_status vbatched_launch(rocblas_handle h, VbArgs<T> a)
 {
@@ -423,23 +228,18 @@ rocblas_status vbatched_launch(rocblas_handle h, VbArgs<T> a)
     const int wgs = int(std::min<int64_t>(int64_t(a.batch) * gmax,
                                           int64_t(h->num_cus) * g_vb_tuning.panel_wgs_per_cu));
     hipLaunchKernelGGL(vb_panel_kernel<T>, dim3(wgs), dim3(panel::PNT), lds, h->stream, a);
-    if((st = launch_status()) != rocblas_status_success)
-        return st;
-    const int waves = std::min(a.batch, h->num_cus * g_vb_tuning.waves_per_cu);
-    hipLaunchKernelGGL(vb_solve_kernel<T>, dim3(waves), dim3(WAVE), 0, h->stream, a);
     return launch_status();
 }
 
 // ---------------------------------------------------------------------------
 // Workspace that needs no allocation (rocblas_amd_dtrsm_vbatched_noalloc):
-// the plan's counters, large-instance list and work items for up to
+// the plan's counters and work items for up to
 // kVbNoAllocMaxBatch instances, in a file-scope device array.  Host code
 // never passes the array's name to a kernel: its address on each device is
 // looked up once (hipGetSymbolAddress, with that device current) and kept.
 // ---------------------------------------------------------------------------
 constexpr int    kVbNoAllocMaxBatch = 65536;
-constexpr size_t kVbScratchBytes    = 64 + size_t(kVbNoAllocMaxBatch) * sizeof(int)
-                                   + size_t(kVbNoAllocMaxBatch) * 8 * sizeof(int);
+constexpr size_t kVbScratchBytes    = 64 + size_t(kVbNoAllocMaxBatch) * 8 * sizeof(int);
 static_assert(kVbScratchBytes % 64 == 0, "scratch sections are 64-byte rounded");
 
 static __device__ __align__(256) unsigned char scratch[kVbScratchBytes];
@@ -472,21 +272,18 @@ static void* vb_scratch_ptr(int device)
 
 namespace
 {
-// Workspace bytes of a batch: counters, then the large-instance list, then
-// the panel work items (8 per instance).
+// Workspace bytes of a batch: counters, then the panel work items (8 per
+// instance).
 inline size_t vb_workspace_bytes(rocblas_int batch_count)
 {
-    return 64 + roundup_ws(size_t(batch_count) * sizeof(int))
-           + roundup_ws(size_t(batch_count) * 8 * sizeof(int));
+    return 64 + roundup_ws(size_t(batch_count) * 8 * sizeof(int));
 }
 
 template <typename T>
 void vb_carve(VbArgs<T>& a, void* wsp)
 {
     a.counter = static_cast<int*>(wsp);
-    a.order   = reinterpret_cast<int*>(static_cast<char*>(wsp) + 64);
-    a.items   = reinterpret_cast<int*>(static_cast<char*>(wsp) + 64
-                                     + roundup_ws(size_t(a.batch) * sizeof(int)));
+    a.items   = reinterpret_cast<int*>(static_cast<char*>(wsp) + 64);
 }
 
 template <typename T>
@@ -556,7 +353,6 @@ rocblas_status vb_entry(bool               noalloc,
         a.alpha_host = *alpha;
     else
         a.alpha_dev = alpha;
-    a.nb16_max_k = g_vb_tuning.nb16_max_k;
     for(int b = 0; b < kVbPanelBuckets; ++b)
         a.groups[b] = g_vb_tuning.groups[b];
     vb_carve(a, wsp);
@@ -610,7 +406,6 @@ rocblas_status vbatched_cases(rocblas_handle     handle,
     a.alpha_dev  = nullptr;
     a.flags_arr  = flags;
     a.alpha_arr  = alpha;
-    a.nb16_max_k = g_vb_tuning.nb16_max_k;
     for(int b = 0; b < kVbPanelBuckets; ++b)
         a.groups[b] = g_vb_tuning.groups[b];
     vb_carve(a, lease.ptr);
@@ -673,16 +468,11 @@ rocblas_status rocblas_amd_vbatched_set_bucket_tuning(rocblas_int bucket, rocbla
     return rocblas_status_success;
 }
 
-rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int nb16_max_k,
-                                               rocblas_int waves_per_cu,
-                                               rocblas_int panel_groups)
+rocblas_status rocblas_amd_vbatched_set_tuning(rocblas_int panel_groups)
 {
-    if(nb16_max_k < 0 || waves_per_cu < 0 || waves_per_cu > 32 || panel_groups < 0
-       || panel_groups > 8)
+    if(panel_groups < 0 || panel_groups > 8)
         return rocblas_status_invalid_value;
     const VbTuning def;
-    g_vb_tuning.nb16_max_k   = nb16_max_k ? nb16_max_k : def.nb16_max_k;
-    g_vb_tuning.waves_per_cu = waves_per_cu ? waves_per_cu : def.waves_per_cu;
     for(int b = 0; b < kVbPanelBuckets; ++b)
         g_vb_tuning.groups[b] = panel_groups ? panel_groups : def.groups[b];
     return rocblas_status_success;

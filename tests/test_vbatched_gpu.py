@@ -143,6 +143,33 @@ def test_random_against_oracle(h, side, uplo, trans):
         assert eta <= 1e-14, (i, m, n, eta)
 
 
+@pytest.mark.parametrize("side,uplo,trans", [("L", "L", "N"), ("R", "U", "T")])
+def test_large_triangles_against_oracle(h, side, uplo, trans):
+    """Triangles beyond 512 (up to 1100) in the same batch as small ones take
+    the same panel path: agreement with the oracle and backward error."""
+    rng = np.random.default_rng(77 + ALL_CASES.index((side, uplo, trans)))
+    kv = [(600, 20), (1100, 5), (700, 70), (513, 1), (40, 9), (1024, 16)]  # (k, vectors)
+    ms = [k if side == "L" else v for k, v in kv]
+    ns = [v if side == "L" else k for k, v in kv]
+    As = [oracle.diag_dominant_triangular(k, uplo, rng) for k, _ in kv]
+    Bs = [np.asfortranarray(rng.uniform(-1, 1, (m, n))) for m, n in zip(ms, ns)]
+    say("vbatched large", side, uplo, trans)
+    P = VbProblem(side, ms, ns, As, Bs)
+    alpha = 0.75
+    assert P.run(h, side, uplo, trans, "N", alpha) == S.success
+    torch.cuda.synchronize()
+    for i, ((k, _), m, n) in enumerate(zip(kv, ms, ns)):
+        ref = Bs[i].copy(order="F")
+        oracle.trsm(side, uplo, trans, "N", m, n, alpha, As[i], ref)
+        got = P.result(i)
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (i, m, n)
+        op = op_dense(As[i], uplo, trans, "N", k)
+        R = alpha * Bs[i] - (op @ got if side == "L" else got @ op)
+        n1 = lambda M: np.abs(M).sum(axis=0).max()
+        eta = n1(R) / (n1(op) * n1(got) + abs(alpha) * n1(Bs[i]))
+        assert eta <= 1e-14, (i, m, n, eta)
+
+
 def test_matches_single_instance_entry_and_skips_malformed(h):
     """Each instance matches rocblas_dtrsm on it to 1e-13; instances with an
     empty size or a too-small leading dimension are left untouched; alpha = 0
