@@ -221,6 +221,38 @@ def test_float32(h, side, uplo, trans):
 
 # --------------------------------------------------------- batched forms --
 
+@pytest.mark.parametrize("side,uplo,trans", [("L", "L", "N"), ("L", "U", "T"), ("R", "L", "T"),
+                                              ("R", "U", "N")])
+def test_small_to_panel_handover_batched_matches_single(h, side, uplo, trans):
+    """Triangles of 33..64 rows go to the small LDS kernel or to the panel
+    solve by k and the vector count (48 < k, or 32 < k with at most 16
+    vectors): shapes on both sides of each boundary, a strided batch of three
+    against the single-instance calls bit for bit, and against the oracle."""
+    rng = np.random.default_rng(4040 + ALL_CASES.index((side, uplo, trans)))
+    for (k, nv) in [(33, 16), (33, 17), (48, 16), (48, 17), (49, 64), (64, 1), (64, 33),
+                    (40, 70)]:
+        m, n = (k, nv) if side == "L" else (nv, k)
+        say("handover", side, uplo, trans, m, n)
+        batch, alpha = 3, -0.75
+        As = [oracle.diag_dominant_triangular(k, uplo, rng) for _ in range(batch)]
+        Bs = [np.asfortranarray(rng.uniform(-1, 1, (m, n))) for _ in range(batch)]
+        dA = torch.from_numpy(np.concatenate([A.ravel(order="F") for A in As])).cuda()
+        dB = torch.from_numpy(np.concatenate([B.ravel(order="F") for B in Bs])).cuda()
+        st = rb.trsm_strided_batched(h, side, uplo, trans, "N", m, n, alpha, dA.data_ptr(), k,
+                                     k * k, dB.data_ptr(), m, m * n, batch)
+        assert st == S.success
+        torch.cuda.synchronize()
+        out = dB.cpu().numpy()
+        for b in range(batch):
+            st1, single = gpu_trsm(h, side, uplo, trans, "N", m, n, alpha, As[b], Bs[b])
+            assert st1 == S.success
+            got = np.asfortranarray(out[b * m * n:(b + 1) * m * n].reshape(n, m).T)
+            assert np.array_equal(got, single), (k, nv, b)
+            ref = Bs[b].copy(order="F")
+            oracle.trsm(side, uplo, trans, "N", m, n, alpha, As[b], ref)
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (k, nv, b)
+
+
 @pytest.mark.parametrize("side,uplo,trans", ALL_CASES)
 def test_strided_and_pointer_batched_match_single(h, side, uplo, trans):
     rng = np.random.default_rng(404)
