@@ -322,17 +322,20 @@ ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_vbatched_set_bucket_tuning(rocblas
  * and reports whether a kernel of an earlier call on this handle gave up one
  * of its bounded waits, which leaves that call's result wrong:
  * rocblas_status_internal_error if so (the report is then cleared), else
- * success.  Only the chained single-instance solve waits (on other workgroups
- * of its own launch, bounded to about a second).  The next call on the handle
- * that launches a solve also reports and clears it, returning
+ * success.  Two kernels wait on other workgroups of their own launch, each
+ * wait bounded to about a second: the chained single-instance solve, and the
+ * triangular products of rocblas_?trtri whose K ranges are split over several
+ * workgroups.  The next call on the handle that launches a solve or an
+ * inversion also reports and clears it, returning
  * rocblas_status_internal_error without doing any work.  No counterpart in
  * the reference, whose waits are unbounded. */
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_get_async_status(rocblas_handle handle);
 
 /* Test hook for the failure path above: with enable != 0, every chained
- * launch on this handle gives up each wait after a few polls (its result is
- * wrong and the error is raised; every kernel still runs to its end).  0
- * restores normal launches.  No counterpart in the reference. */
+ * launch and every split trtri product on this handle gives up each wait
+ * after a few polls (its result is wrong and the error is raised; every
+ * kernel still runs to its end).  0 restores normal launches.  No
+ * counterpart in the reference. */
 ROCBLAS_AMD_EXPORT rocblas_status rocblas_amd_set_chain_test_timeout(rocblas_handle handle,
                                                                      int            enable);
 

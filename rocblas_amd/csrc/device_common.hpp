@@ -281,6 +281,18 @@ struct GemmArgsT
     int                tri;          // GemmTri bits, 0 = dense operands
     int                tiles_m, tiles_n;
     int                batch;
+    // Split of the K range over several workgroups per tile (FP64 DMA kernel,
+    // triangular operands).  split_ok: the caller allows it (the handle's
+    // counters and error word are then used).  The launcher sets the rest:
+    // kc = the longest K piece (0 = no split), the counters (one per tile and
+    // instance, zero between launches), the error word a given-up wait
+    // raises, and the wait controls.
+    int                split_ok;
+    int                kc;
+    unsigned*          split_cnt;
+    unsigned*          split_err;
+    int                spin_limit;
+    unsigned           need_bias;
 };
 
 template <typename T>

@@ -359,10 +359,57 @@ struct DmaOperand
     }
 };
 
-template <int BK, int NS, int OPA, int OPB, bool TRI>
+// Wait of a split piece: thread 0 polls the tile's counter (agent scope)
+// until it reaches `need`, at most spin_limit times; a wait that gives up
+// raises the error word (the sum is then wrong, and the host reports it) and
+// goes on.  Every wavefront then acquires what the earlier pieces wrote.
+__device__ __forceinline__ void split_wait(unsigned* cnt, unsigned need, int spin_limit,
+                                           unsigned* err)
+{
+    if(threadIdx.x == 0)
+    {
+        for(int it = 0;; ++it)
+        {
+            if(__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need)
+                break;
+            if(it >= spin_limit)
+            {
+                __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// A split piece has stored its part of C: every wavefront releases its
+// stores, then one thread counts the piece finished; whichever of the tile's
+// `parts` pieces finishes last (normally the last piece) clears the counter
+// for the next launch, after every other piece has counted itself.
+__device__ __forceinline__ void split_signal(unsigned* cnt, unsigned parts)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        const unsigned old
+            = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if(old == parts - 1)
+            __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// TRI: 0 for dense operands, else the one GemmTri mask bit of the
+// triangular operand (GT_A_LOWER ... GT_B_UPPER), fixed at compile time so
+// that the masks are plain selects in the K loop; GT_UNIT is read from a.tri.
+template <int BK, int NS, int OPA, int OPB, int TRI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     gemm_dma_kernel(const GemmArgsT<double, double, double> a)
 {
+    constexpr bool TRI_A  = TRI == GT_A_LOWER || TRI == GT_A_UPPER;
+    constexpr bool TO_END = TRI == GT_A_LOWER || TRI == GT_B_UPPER; // range ends at the diagonal
     static_assert((BK == 8 || BK == 16) && NS >= 2 && NS <= 3, "configuration");
     using OA             = DmaOperand<BK, OPA == 0>;
     using OB             = DmaOperand<BK, OPB == 1>;
@@ -384,37 +431,53 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     // row-panel groups, except with a triangular operand, whose row (column)
     // panels carry unequal work: contiguous runs would give one XCD all the
     // long panels (1.9x the mean at 16384^2), so dispatch order is kept there
-    // (round-robin over the XCDs)
+    // (round-robin over the XCDs).  A split launch numbers its workgroups
+    // piece-major: piece p of tile t is workgroup t + p * tiles.
     const int total = a.tiles_m * a.tiles_n;
     int       bid   = blockIdx.x;
-    if constexpr(!TRI)
+    int       piece = 0;
+    if constexpr(TRI != 0)
+    {
+        piece = bid / total;
+        bid -= piece * total;
+    }
+    else
     {
         const int q = total / 8, r = total % 8;
         const int x = bid % 8, l = bid / 8;
         bid         = x * q + min(x, r) + l;
     }
+    const int     tile   = bid;
     constexpr int GROUP  = 8;
     const int     grp    = bid / (GROUP * a.tiles_n);
     const int     fm     = grp * GROUP;
     const int     gsz    = min(a.tiles_m - fm, GROUP);
     const int     m0     = (fm + (bid % gsz)) * 128;
     const int     n0     = ((bid % (GROUP * a.tiles_n)) / gsz) * 128;
-    // A triangular operand (square, K x K): this tile takes only the K range
-    // in which the operand is dense, short of the diagonal block of its
-    // rows (A) or columns (B); launch_gemm adds the diagonal blocks with the
-    // masked kernel.  m0 and n0 are multiples of 128, so the range stays a
-    // whole number of K steps.
-    int kbeg = 0, kend = a.K;
-    if constexpr(TRI)
+    // A triangular operand (square, K x K): this tile takes the K range in
+    // which the operand is non-zero for its rows (A) or columns (B), up to and
+    // including their diagonal block [dlo, dhi).  With a split (kc > 0) the
+    // range is cut into P = ceil(len / kc) near-equal pieces, whole K steps
+    // each, and this workgroup takes piece `piece` (none past P).
+    int kbeg = 0, kend = a.K, parts = 1, dlo = 0, dhi = 0;
+    if constexpr(TRI != 0)
     {
-        if(a.tri & GT_A_LOWER)
-            kend = min(kend, m0);
-        if(a.tri & GT_A_UPPER)
-            kbeg = max(kbeg, m0 + 128);
-        if(a.tri & GT_B_LOWER)
-            kbeg = max(kbeg, n0 + 128);
-        if(a.tri & GT_B_UPPER)
-            kend = min(kend, n0);
+        dlo = TRI_A ? m0 : n0;
+        dhi = min(a.K, dlo + 128);
+        if constexpr(TO_END)
+            kend = dhi;
+        else
+            kbeg = dlo;
+        const int len = max(kend - kbeg, 0);
+        if(a.kc > 0 && len > a.kc)
+        {
+            parts         = (len + a.kc - 1) / a.kc;
+            const int pl  = ((len + parts - 1) / parts + BK - 1) / BK * BK;
+            kbeg          = min(kend, kbeg + piece * pl);
+            kend          = min(kend, kbeg + pl);
+        }
+        if(piece >= parts)
+            return;
     }
     const int ktiles = kend > kbeg ? (kend - kbeg) / BK : 0;
 
@@ -430,7 +493,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         // tiles past the last one re-read the last tile (in bounds) into a
         // stage nobody reads any more, so every step waits the same way
         auto issue = [&](int kt, int st) {
-            const int64_t k0 = (TRI ? kbeg : 0) + int64_t(min(kt, ktiles - 1)) * BK;
+            const int64_t k0 = (TRI != 0 ? kbeg : 0) + int64_t(min(kt, ktiles - 1)) * BK;
 #pragma unroll
             for(int g = 0; g < PER; ++g)
                 dma16(OA::src(Ap, la, g, w, k0, a.lda, a.M), &As[st][OA::dst(g, w)]);
@@ -446,6 +509,36 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
             for(int j = 0; j < 4; ++j)
                 acc[i][j] = d4_t{0, 0, 0, 0};
 
+        // A triangular operand's fragments are zeroed outside its triangle (a
+        // select, so whatever the other triangle holds never enters) and set
+        // to one on a unit diagonal.  Every K step applies the test: outside
+        // the diagonal block it never fires, since the tile's K range ends
+        // (starts) at that block where the zeros lie above (below) it.
+        // A K step of the diagonal block: the triangular operand's stage is
+        // cleared outside its triangle (whatever the other triangle holds
+        // never enters) and set to one on a unit diagonal, 4 elements per
+        // thread, before any wavefront reads it.
+        auto mask_stage = [&](int st, int k0) {
+            double* stage = TRI_A ? As[st] : Bs[st];
+            // opaque copy of the thread index: keeps the compiler from
+            // holding these addresses over the whole K loop (registers the
+            // kernel lacks)
+            int t = tid;
+            asm volatile("" : "+v"(t));
+            const int x    = t & 127;
+            const int unit = a.tri & GT_UNIT;
+#pragma unroll
+            for(int q = 0; q < 4; ++q)
+            {
+                const int k = (t >> 7) * 4 + q;
+                const int e = k0 + k - ((TRI_A ? m0 : n0) + x); // k - row (A) / column (B)
+                const int o = TRI_A ? OA::at(k, x) : OB::at(k, x);
+                if(TO_END ? e > 0 : e < 0)
+                    stage[o] = 0.0;
+                else if(unit && e == 0)
+                    stage[o] = 1.0;
+            }
+        };
         auto compute = [&](int st) {
             const double* as = As[st];
             const double* bs = Bs[st];
@@ -475,6 +568,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
             wait_vmcnt<(NS - 2) * LOADS>();
             barrier_no_fence();
             issue(kt + NS - 1, (st + NS - 1) % NS);
+            if constexpr(TRI != 0)
+            {
+                const int k0 = kbeg + kt * BK;
+                if(k0 >= dlo && k0 < dhi)
+                {
+                    mask_stage(st, k0);
+                    barrier_no_fence();
+                }
+            }
             compute(st);
         };
 
@@ -493,6 +595,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
             wait_vmcnt<0>(); // the tiles past the end
         }
 
+        // Split pieces of a tile add into C one after another, in piece
+        // order, so the sum is the same whatever order they finish in: piece
+        // p waits until p pieces have finished (a bounded poll by one thread,
+        // then every wavefront acquires), piece 0 applies beta.  Earlier
+        // pieces have lower workgroup ids and were dispatched first, so every
+        // wait ends.
+        unsigned* cnt = nullptr;
+        if constexpr(TRI != 0)
+        {
+            if(parts > 1)
+            {
+                cnt = a.split_cnt + (int64_t(b) * total + tile);
+                if(piece > 0)
+                    split_wait(cnt, unsigned(piece) + a.need_bias, a.spin_limit, a.split_err);
+            }
+        }
+
         // epilogue: as gemm_kernel's
 #pragma unroll
         for(int i = 0; i < 4; ++i)
@@ -508,11 +627,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
                     {
                         double& cref = Cp[row + int64_t(col) * a.ldc];
                         double  v    = a.alpha * acc[i][j][r];
-                        if(a.beta != 0.0)
+                        if(piece > 0)
+                            v = cref + v;
+                        else if(a.beta != 0.0)
                             v = fma(a.beta, cref, v);
                         cref = v;
                     }
                 }
+        }
+        if constexpr(TRI != 0)
+        {
+            if(parts > 1)
+                split_signal(cnt, unsigned(parts));
         }
         __syncthreads(); // LDS reuse across batch iterations
     }
@@ -736,77 +862,26 @@ static bool gemm_dma_eligible(const GemmArgsT<float, float, float>& a, int bk)
            && al16(a.B.base + a.B.offset) && a.A.stride % 4 == 0 && a.B.stride % 4 == 0;
 }
 
-// The DMA kernel's conditions: plain FP64 operands, 16-byte aligned rows; a
-// triangular operand must be square (K x K) with one instance, and the
-// kernel then leaves its diagonal blocks to the caller.
+// The DMA kernel's conditions: plain FP64 operands, 16-byte aligned rows; at
+// most one triangular operand, square (K x K), and then at most one of the
+// two operands transposed.
 static bool gemm_dma_eligible(const GemmArgsT<double, double, double>& a, int bk)
 {
     auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
     // an x-contiguous operand is loaded two rows of op() at a time
     const bool a_pairs = a.opA == 1 || (a.M >= 2 && a.M % 2 == 0);
     const bool b_pairs = a.opB == 0 || (a.N >= 2 && a.N % 2 == 0);
+    const int  kind    = a.tri & (GT_A_LOWER | GT_A_UPPER | GT_B_LOWER | GT_B_UPPER);
     const bool tri_ok  = !a.tri
-                        || (a.batch == 1 && !(a.opA == 1 && a.opB == 1)
+                        || ((kind == GT_A_LOWER || kind == GT_A_UPPER || kind == GT_B_LOWER
+                             || kind == GT_B_UPPER)
+                            && !(a.opA == 1 && a.opB == 1)
                             && (!(a.tri & (GT_A_LOWER | GT_A_UPPER)) || a.M == a.K)
                             && (!(a.tri & (GT_B_LOWER | GT_B_UPPER)) || a.N == a.K));
     return tri_ok && a_pairs && b_pairs && a.K >= bk && a.K % bk == 0 && a.lda % 2 == 0
            && a.ldb % 2 == 0 && !a.A.arr && !a.B.arr
            && al16(a.A.base + a.A.offset) && al16(a.B.base + a.B.offset) && a.A.stride % 2 == 0
            && a.B.stride % 2 == 0;
-}
-
-template <typename TA, typename TB, typename TC>
-static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a,
-                                      bool allow_dma);
-
-// The diagonal blocks of a triangular operand that the DMA kernel left out:
-// C[block] += alpha op(A)[block] op(B)[block] over the 128-wide blocks of K,
-// on the masked kernel (the full blocks as one strided batch, then the
-// partial one).  beta is already applied.  The DMA kernel is never chosen
-// here: a single diagonal block would be eligible again and lead back here.
-static rocblas_status gemm_tri_diagonal(rocblas_handle h, const GemmArgsT<double, double, double>& a)
-{
-    const bool    onA  = a.tri & (GT_A_LOWER | GT_A_UPPER);
-    const int     full = a.K / 128, part = a.K % 128;
-    for(int pass = 0; pass < 2; ++pass)
-    {
-        const int nblk = pass == 0 ? full : (part ? 1 : 0);
-        const int bs   = pass == 0 ? 128 : part;
-        if(!nblk)
-            continue;
-        const int64_t k0 = pass == 0 ? 0 : int64_t(full) * 128;
-        GemmArgsT<double, double, double> d = a;
-        d.beta  = 1.0;
-        d.K     = bs;
-        d.batch = nblk;
-        d.skip_if_zero = a.skip_if_zero;
-        // the triangular operand's diagonal block k0 + 128 b (either storage
-        // order), the dense operand's matching K rows / columns, C's block
-        const int64_t tstep = 128 * (int64_t(onA ? a.lda : a.ldb) + 1);
-        if(onA)
-        {
-            d.M = bs;
-            d.A = BatchPtr<const double>{nullptr, a.A.base, tstep,
-                                         a.A.offset + k0 * (int64_t(a.lda) + 1)};
-            const int64_t bst = a.opB == 0 ? 128 : 128 * int64_t(a.ldb);
-            d.B = BatchPtr<const double>{nullptr, a.B.base, bst, a.B.offset + k0 / 128 * bst};
-            d.C = BatchPtr<double>{nullptr, a.C.base, 128, a.C.offset + k0};
-        }
-        else
-        {
-            d.N = bs;
-            d.B = BatchPtr<const double>{nullptr, a.B.base, tstep,
-                                         a.B.offset + k0 * (int64_t(a.ldb) + 1)};
-            const int64_t ast = a.opA == 0 ? 128 * int64_t(a.lda) : 128;
-            d.A = BatchPtr<const double>{nullptr, a.A.base, ast, a.A.offset + k0 / 128 * ast};
-            d.C = BatchPtr<double>{nullptr, a.C.base, 128 * int64_t(a.ldc),
-                                   a.C.offset + k0 * a.ldc};
-        }
-        const rocblas_status st = launch_gemm_sel(h, d, false);
-        if(st != rocblas_status_success)
-            return st;
-    }
-    return rocblas_status_success;
 }
 
 // MFMA type of an operand pair: FP64 only when both operands are FP64.
@@ -881,7 +956,7 @@ static rocblas_status launch_gemm_cfg(rocblas_handle h, GemmArgsT<TA, TB, TC> a)
     return launch_status();
 }
 
-template <int BK, int NS, bool TRI>
+template <int BK, int NS, int TRI>
 static void launch_gemm_dma_ops(rocblas_handle h, const GemmArgsT<double, double, double>& d,
                                 dim3 grid)
 {
@@ -894,7 +969,7 @@ static void launch_gemm_dma_ops(rocblas_handle h, const GemmArgsT<double, double
     else if(d.opA == 0 && d.opB == 1)
         hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 0, 1, TRI>), grid, dim3(256), 0, h->stream,
                            d);
-    else if constexpr(!TRI)
+    else if constexpr(TRI == 0)
         hipLaunchKernelGGL((gemm_dma_kernel<BK, NS, 1, 1, TRI>), grid, dim3(256), 0, h->stream,
                            d);
 }
@@ -928,12 +1003,18 @@ static void launch_gemm_dma(rocblas_handle h, const GemmArgsT<double, double, do
     // triangular operands: the 8-deep kernel only (the 16-deep one needs
     // more registers than its two waves per SIMD allow once the per-tile K
     // range is added, and would spill)
-    if(d.tri)
-        launch_gemm_dma_ops<8, 3, true>(h, d, grid);
-    else if(bk == 16)
-        launch_gemm_dma_ops<16, 2, false>(h, d, grid);
-    else
-        launch_gemm_dma_ops<8, 3, false>(h, d, grid);
+    switch(d.tri & (GT_A_LOWER | GT_A_UPPER | GT_B_LOWER | GT_B_UPPER))
+    {
+    case GT_A_LOWER: launch_gemm_dma_ops<8, 3, GT_A_LOWER>(h, d, grid); break;
+    case GT_A_UPPER: launch_gemm_dma_ops<8, 3, GT_A_UPPER>(h, d, grid); break;
+    case GT_B_LOWER: launch_gemm_dma_ops<8, 3, GT_B_LOWER>(h, d, grid); break;
+    case GT_B_UPPER: launch_gemm_dma_ops<8, 3, GT_B_UPPER>(h, d, grid); break;
+    default:
+        if(bk == 16)
+            launch_gemm_dma_ops<16, 2, 0>(h, d, grid);
+        else
+            launch_gemm_dma_ops<8, 3, 0>(h, d, grid);
+    }
 }
 
 // ROCBLAS_AMD_GEMM_DMA (for A/B comparisons): 0 keeps every GEMM on
@@ -953,10 +1034,78 @@ static int gemm_dma_mode()
     return mode;
 }
 
+// ROCBLAS_AMD_GEMM_SPLIT (for A/B comparisons): 0 never splits K; otherwise
+// the shortest K piece a split may make (a multiple of 128, default 128).
+static int gemm_split_min()
+{
+    static const int v = [] {
+        const char* e = std::getenv("ROCBLAS_AMD_GEMM_SPLIT");
+        if(!e)
+            return 128;
+        const int x = std::atoi(e);
+        return x <= 0 ? 0 : std::max(128, x / 128 * 128);
+    }();
+    return v;
+}
+
+// A triangular-operand product with few 128 x 128 tiles on the DMA kernel,
+// each tile's K range cut into pieces so that the launch fills the chip's two
+// workgroups per CU: the longest piece kc is the shortest multiple of 128
+// (not below the minimum) for which the pieces of every instance fit in
+// those slots; without such a kc the launch is not split.  The pieces of a
+// tile add into C in piece order (gemm_dma_kernel), so the result depends
+// only on the shapes and the CU count, not on timing.
+static rocblas_status launch_gemm_dma_split(rocblas_handle h, const GemmArgsT<double, double, double>& a)
+{
+    GemmArgsT<double, double, double> d = a;
+    d.tiles_m                           = (a.M + 127) / 128;
+    d.tiles_n                           = (a.N + 127) / 128;
+    const int64_t tiles                 = int64_t(d.tiles_m) * d.tiles_n;
+    const int64_t slots                 = 2 * int64_t(h->num_cus);
+    const bool    onA                   = a.tri & (GT_A_LOWER | GT_A_UPPER);
+    const bool    toK                   = a.tri & (GT_A_LOWER | GT_B_UPPER); // range ends at the diagonal
+    // the K range of the tiles in row (A) / column (B) panel t, as the kernel takes it
+    auto len_of = [&](int t) {
+        const int lo = t * 128, hi = std::min(a.K, lo + 128);
+        return toK ? hi : a.K - lo;
+    };
+    const int panels = onA ? d.tiles_m : d.tiles_n;
+    const int other  = onA ? d.tiles_n : d.tiles_m;
+    auto parts = [&](int len, int kc) { return len > kc ? (len + kc - 1) / kc : 1; };
+    d.kc      = 0;
+    int pmax  = 1;
+    const int kmin = gemm_split_min();
+    if(kmin && split_counters(h) && h->async_error && tiles * a.batch <= kSplitCounters
+       && a.batch <= 65535)
+        for(int kc = kmin; kc < a.K; kc += 128)
+        {
+            int64_t n  = 0;
+            int     pm = 1;
+            for(int t = 0; t < panels; ++t)
+            {
+                const int p = parts(len_of(t), kc);
+                n += int64_t(p) * other;
+                pm = std::max(pm, p);
+            }
+            if(n * a.batch <= slots)
+            {
+                if(pm > 1)
+                    d.kc = kc, pmax = pm;
+                break;
+            }
+        }
+    d.split_cnt  = split_counters(h);
+    d.split_err  = h->async_error;
+    d.spin_limit = h->chain_debug_timeout ? 64 : (1 << 20);
+    d.need_bias  = h->chain_debug_timeout ? (1u << 30) : 0u;
+    const dim3 grid(unsigned(tiles * pmax), grid_batch(a.batch));
+    launch_gemm_dma(h, d, grid, 8);
+    return launch_status();
+}
+
 // Pick the largest tile that still gives every CU work.
 template <typename TA, typename TB, typename TC>
-static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a,
-                                      bool allow_dma)
+static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
 {
     if(a.M <= 0 || a.N <= 0)
         return rocblas_status_success;
@@ -970,6 +1119,18 @@ static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, 
     if(a.M <= 32)
         return tiles(32, 128) >= cus ? launch_gemm_cfg<TA, TB, TC, 32, 128, 16, 1, 4>(h, a)
                                      : launch_gemm_cfg<TA, TB, TC, 32, 32, 16, 2, 2>(h, a);
+    // one triangular product whose tiles fill between a quarter and all of
+    // the chip's workgroup slots: the DMA kernel with the K ranges split,
+    // where the caller allows it (fewer tiles: the pieces would be too short
+    // for the DMA kernel's per-step latency, and the masked kernels below run
+    // faster)
+    if constexpr(std::is_same_v<TA, double> && std::is_same_v<TB, double>
+                 && std::is_same_v<TC, double>)
+    {
+        if(a.tri && a.split_ok && a.batch == 1 && gemm_dma_mode() && tiles(128, 128) * 2 >= cus
+           && tiles(128, 128) < 2 * cus && gemm_dma_eligible(a, 8))
+            return launch_gemm_dma_split(h, a);
+    }
     if(tiles(128, 128) >= cus)
     {
         // FP64 runs best with K steps of 8 (two waves per SIMD), FP32 with 16
@@ -979,7 +1140,7 @@ static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, 
         {
             const int mode = gemm_dma_mode();
             const int bk   = mode != 8 && a.K % 32 == 0 ? 32 : 16;
-            if(allow_dma && mode && gemm_dma_eligible(a, bk))
+            if(mode && gemm_dma_eligible(a, bk))
             {
                 GemmArgsT<TA, TB, TC> d = a;
                 d.tiles_m               = (a.M + 127) / 128;
@@ -994,17 +1155,15 @@ static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, 
         {
             const int mode = gemm_dma_mode();
             const int bk   = mode == 16 && a.K % 16 == 0 && !a.tri ? 16 : 8;
-            if(allow_dma && mode && gemm_dma_eligible(a, bk))
+            if(mode && gemm_dma_eligible(a, bk))
             {
                 GemmArgsT<TA, TB, TC> d = a;
                 d.tiles_m               = (a.M + 127) / 128;
                 d.tiles_n               = (a.N + 127) / 128;
                 const dim3 grid(unsigned(int64_t(d.tiles_m) * d.tiles_n), grid_batch(a.batch));
+                d.kc                    = 0;
                 launch_gemm_dma(h, d, grid, bk);
-                const rocblas_status st = launch_status();
-                if(st != rocblas_status_success || !a.tri)
-                    return st;
-                return gemm_tri_diagonal(h, a);
+                return launch_status();
             }
         }
         if constexpr(std::is_same_v<mfma_t<TA, TB>, double>)
@@ -1020,7 +1179,7 @@ static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, 
 template <typename TA, typename TB, typename TC>
 rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
 {
-    return launch_gemm_sel(h, a, true);
+    return launch_gemm_sel(h, a);
 }
 
 template rocblas_status launch_gemm(rocblas_handle, const GemmArgsT<float, float, float>&);

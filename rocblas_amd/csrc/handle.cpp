@@ -140,10 +140,13 @@ try
         }
         h->ws_size = want;
     }
-    // completion counters of the chained solve, zero between launches, and
-    // the host-visible error word its bounded waits raise
-    if(hipMalloc(&h->chain_flags, rbamd::kChainFlagsBytes) != hipSuccess
-       || hipMemset(h->chain_flags, 0, rbamd::kChainFlagsBytes) != hipSuccess
+    // completion counters of the chained solve and of the split-K GEMM, zero
+    // between launches, and the host-visible error word their bounded waits
+    // raise
+    constexpr size_t flag_bytes
+        = rbamd::kChainFlagsBytes + rbamd::kSplitCounters * sizeof(unsigned);
+    if(hipMalloc(&h->chain_flags, flag_bytes) != hipSuccess
+       || hipMemset(h->chain_flags, 0, flag_bytes) != hipSuccess
        || hipHostMalloc(reinterpret_cast<void**>(&h->async_error), sizeof(unsigned int),
                         hipHostMallocCoherent)
               != hipSuccess

@@ -41,13 +41,15 @@ struct _rocblas_handle
     bool                 size_query    = false;
     size_t               query_size    = 0;
     // completion counters of the chained solve (chain_trsm.hip): one per
-    // panel, zero between launches
+    // panel, zero between launches; followed in the same allocation by the
+    // tile counters of the split-K GEMM (kSplitCounters, zero between launches)
     void*                chain_flags   = nullptr;
     // host-visible word a kernel raises when one of its bounded waits gave
     // up; read and cleared by the next launching call and by
     // rocblas_amd_get_async_status
     unsigned int*        async_error   = nullptr;
-    // test hook: chained launches give up every wait after a few polls
+    // test hook: chained launches and split-K GEMMs give up every wait after
+    // a few polls
     bool                 chain_debug_timeout = false;
     // ROCBLAS_LAYER bits read at creation and the streams they opened
     int                  layer_mode     = 0;
@@ -60,6 +62,16 @@ namespace rbamd
 {
 // Bytes of a handle's chain_flags buffer.
 constexpr size_t kChainFlagsBytes = 4096;
+// Tile counters of the split-K GEMM after them: one per (instance, tile) of a
+// split launch.
+constexpr int kSplitCounters = 4096;
+
+inline unsigned* split_counters(rocblas_handle h)
+{
+    return h->chain_flags ? reinterpret_cast<unsigned*>(static_cast<char*>(h->chain_flags)
+                                                        + kChainFlagsBytes)
+                          : nullptr;
+}
 
 // Round a workspace request the way the reference does (64-byte chunks,
 // library/src/include/handle.hpp:56-61).

@@ -117,6 +117,7 @@ rocblas_status trtri_double_step(rocblas_handle    h,
     GemmArgs<T> g{};
     g.beta         = T(0);
     g.skip_if_zero = nullptr;
+    g.split_ok     = 1; // products with few tiles may split their K ranges
     g.batch        = multi ? groups : batch;
     g.lda          = lda;
     g.ldb          = ldx;
@@ -211,6 +212,9 @@ rocblas_status trtri_core(rocblas_handle    handle,
                           int               batch_count,
                           T*                scratch)
 {
+    // a split product of an earlier call gave up a wait: report it here
+    if(rocblas_status st = take_async_error(handle); st != rocblas_status_success)
+        return st;
     BatchPtr<const T> Asrc = A;
     int               ldas = lda;
     if(scratch)

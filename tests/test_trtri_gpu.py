@@ -229,3 +229,104 @@ def test_blocked_batched_matches_single(h):
         for b in range(batch):
             got = dIs[b * n * n:(b + 1) * n * n]
             assert torch.allclose(got, singles[b], rtol=0, atol=1e-13), b
+
+
+def _bidiagonal(n, uplo, diag_value, poison):
+    """Column-major I minus the unit sub (super) diagonal on the device, with
+    `diag_value` on the diagonal and `poison` in the opposite triangle; its
+    inverse is the all-ones triangle (with the diagonal taken as one)."""
+    V = torch.eye(n, dtype=torch.float64, device="cuda") * diag_value  # V[col, row]
+    i = torch.arange(n - 1, device="cuda")
+    if uplo == "L":
+        V[i, i + 1] = -1.0
+        V.masked_fill_(torch.ones(n, n, dtype=torch.bool, device="cuda").tril(-1), poison)
+    else:
+        V[i + 1, i] = -1.0
+        V.masked_fill_(torch.ones(n, n, dtype=torch.bool, device="cuda").triu(1), poison)
+    return V.reshape(-1).contiguous()
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_split_products_exact_and_repeatable(h, uplo, diag):
+    """Orders whose doubling products run on the DMA kernel with the diagonal
+    blocks of the triangular factors masked in the same kernel, at n = 4096
+    with the top level's K ranges split over several workgroups (even leading
+    dimensions), and on the masked kernels below (n = 1024, 2048).  The inverse of the bidiagonal
+    I - J is the all-ones triangle: every partial sum is a small integer, so
+    the result is exact whatever the split, and NaN in A's opposite triangle
+    (and, for a unit diagonal, 3 on A's diagonal) must never enter.  A second
+    call gives the same bits."""
+    for n in (1024, 2048, 4096):
+        say("trtri split", uplo, diag, n)
+        dA = _bidiagonal(n, uplo, 3.0 if diag == "U" else 1.0, float("nan"))
+        dI = torch.full((n * n,), 5.0, dtype=torch.float64, device="cuda")
+        assert rb.trtri(h, uplo, diag, n, dA.data_ptr(), n, dI.data_ptr(), n) == S.success
+        torch.cuda.synchronize()
+        ones = torch.ones(n, n, dtype=torch.float64, device="cuda")
+        want = (ones.triu() if uplo == "L" else ones.tril()).reshape(-1)  # [col, row] view
+        assert torch.equal(dI, want), n
+        dI2 = torch.full((n * n,), 5.0, dtype=torch.float64, device="cuda")
+        assert rb.trtri(h, uplo, diag, n, dA.data_ptr(), n, dI2.data_ptr(), n) == S.success
+        torch.cuda.synchronize()
+        assert torch.equal(dI, dI2), n
+        assert h.get_async_status() == S.success
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_split_products_accuracy(h, uplo):
+    """Random well-conditioned triangles at n = 2048 and 4096 (split top level):
+    ||invA T - I||_max on the device, and bitwise equal results on a repeat."""
+    for n in (2048, 4096):
+        say("trtri split accuracy", uplo, n)
+        dA = torch.empty(n * n, dtype=torch.float64, device="cuda")
+        assert rb.synth_tri(h, uplo, "uniform", n, 7, dA.data_ptr(), n, scale=1.0 / n) \
+            == S.success
+        dI = torch.empty_like(dA)
+        dI2 = torch.empty_like(dA)
+        assert rb.trtri(h, uplo, "N", n, dA.data_ptr(), n, dI.data_ptr(), n) == S.success
+        assert rb.trtri(h, uplo, "N", n, dA.data_ptr(), n, dI2.data_ptr(), n) == S.success
+        torch.cuda.synchronize()
+        assert torch.equal(dI, dI2)
+        T = (torch.tril if uplo == "L" else torch.triu)(dA.view(n, n).T)
+        X = dI.view(n, n).T
+        err = float((X @ T - torch.eye(n, dtype=torch.float64, device="cuda")).abs().max())
+        assert err <= 1e-12, (n, err)
+        del dA, dI, dI2, T, X
+        torch.cuda.empty_cache()
+
+
+def test_split_wait_timeout_reported_and_healed(h):
+    """The split products' bounded waits, forced to give up through the
+    handle's test hook (every kernel still runs to its end): the handle reports
+    the error once, the next trtri call returns it without work, and after the
+    hook is cleared the tile counters are back at zero, so the result is the
+    exact one again."""
+    n, uplo = 4096, "L"
+    dA = _bidiagonal(n, uplo, 1.0, 0.0)
+    ones = torch.ones(n, n, dtype=torch.float64, device="cuda")
+    want = ones.triu().reshape(-1)
+    ht = GpuHandle()
+    try:
+        assert ht.set_chain_test_timeout(1) == S.success
+        dI = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+        say("trtri split timeout forced")
+        assert rb.trtri(ht, uplo, "N", n, dA.data_ptr(), n, dI.data_ptr(), n) == S.success
+        torch.cuda.synchronize()
+        assert ht.get_async_status() == S.internal_error
+        assert ht.get_async_status() == S.success
+        assert rb.trtri(ht, uplo, "N", n, dA.data_ptr(), n, dI.data_ptr(), n) == S.success
+        torch.cuda.synchronize()
+        assert ht.set_chain_test_timeout(0) == S.success
+        dJ = torch.full((n * n,), 5.0, dtype=torch.float64, device="cuda")
+        assert rb.trtri(ht, uplo, "N", n, dA.data_ptr(), n, dJ.data_ptr(), n) \
+            == S.internal_error
+        torch.cuda.synchronize()
+        assert bool((dJ == 5.0).all())
+        say("trtri split after timeout")
+        assert rb.trtri(ht, uplo, "N", n, dA.data_ptr(), n, dJ.data_ptr(), n) == S.success
+        torch.cuda.synchronize()
+        assert torch.equal(dJ, want)
+        assert ht.get_async_status() == S.success
+    finally:
+        ht.close()
