@@ -403,7 +403,7 @@ __device__ __forceinline__ void split_signal(unsigned* cnt, unsigned parts)
 
 // TRI: 0 for dense operands, else the one GemmTri mask bit of the
 // triangular operand (GT_A_LOWER ... GT_B_UPPER), fixed at compile time so
-// that the masks are plain selects in the K loop; GT_UNIT is read from a.tri.
+// that no mask test branches at run time; GT_UNIT is read from a.tri.
 template <int BK, int NS, int OPA, int OPB, int TRI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     gemm_dma_kernel(const GemmArgsT<double, double, double> a)
@@ -509,11 +509,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
             for(int j = 0; j < 4; ++j)
                 acc[i][j] = d4_t{0, 0, 0, 0};
 
-        // A triangular operand's fragments are zeroed outside its triangle (a
-        // select, so whatever the other triangle holds never enters) and set
-        // to one on a unit diagonal.  Every K step applies the test: outside
-        // the diagonal block it never fires, since the tile's K range ends
-        // (starts) at that block where the zeros lie above (below) it.
         // A K step of the diagonal block: the triangular operand's stage is
         // cleared outside its triangle (whatever the other triangle holds
         // never enters) and set to one on a unit diagonal, 4 elements per
