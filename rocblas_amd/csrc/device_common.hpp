@@ -289,6 +289,12 @@ struct GemmArgsT
     // raises, and the wait controls.
     int                split_ok;
     int                kc;
+    // balanced triangular products (gemm_dma_balanced_kernel): the workspace
+    // of 2 * split_w partial tiles of 128 x 128 (nullptr: not available), its
+    // capacity in tiles, and the launch's workgroup count
+    double*            split_ws;
+    int64_t            split_ws_tiles;
+    int                split_w;
     unsigned*          split_cnt;
     unsigned*          split_err;
     int                spin_limit;
@@ -300,5 +306,10 @@ using GemmArgs = GemmArgsT<T, T, T>;
 
 template <typename TA, typename TB, typename TC>
 rocblas_status launch_gemm(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a);
+
+// Workspace bytes an FP64 product of this shape with one triangular operand
+// (tri: one GemmTri mask bit) uses when its K steps are balanced over the
+// workgroups (split_ok and split_ws set); 0 if it would not be.
+size_t gemm_tri_split_bytes(rocblas_handle h, int M, int N, int K, int tri, int batch);
 
 } // namespace rbamd

@@ -401,38 +401,224 @@ __device__ __forceinline__ void split_signal(unsigned* cnt, unsigned parts)
     }
 }
 
+// One tile job of the DMA kernels: the 128 x 128 tile at (m0, n0) of
+// instance b, over the K range [kbeg, kend), through LDS stages As / Bs.
 // TRI: 0 for dense operands, else the one GemmTri mask bit of the
 // triangular operand (GT_A_LOWER ... GT_B_UPPER), fixed at compile time so
 // that no mask test branches at run time; GT_UNIT is read from a.tri.
+// [dlo, dhi) is the triangular operand's diagonal block, whose K steps are
+// masked in LDS.  The result goes
+//   to `part` when it is given: the raw sums, element v = (4 i + j) 4 + r of
+//     thread t at part[v * 256 + t] (coalesced), for a later in-order sum;
+//   else to C: alpha * sum + beta * C, or, for piece `piece` > 0 of `parts`
+//     pieces of the tile, added to C after the earlier pieces, in piece
+//     order through the tile's counter cnt.
+template <int BK, int NS, int OPA, int OPB, int TRI>
+__device__ __forceinline__ void dma_job(const GemmArgsT<double, double, double>& a,
+                                        double (*As)[DmaOperand<BK, OPA == 0>::ELS],
+                                        double (*Bs)[DmaOperand<BK, OPB == 1>::ELS], int b,
+                                        int m0, int n0, int kbeg, int kend, int dlo, int dhi,
+                                        int piece, int parts, unsigned* cnt, double* part)
+{
+    constexpr bool TRI_A  = TRI == GT_A_LOWER || TRI == GT_A_UPPER;
+    constexpr bool TO_END = TRI == GT_A_LOWER || TRI == GT_B_UPPER; // range ends at the diagonal
+    using OA              = DmaOperand<BK, OPA == 0>;
+    using OB              = DmaOperand<BK, OPB == 1>;
+    constexpr int PER     = BK / 4;  // load instructions per operand per wavefront
+    constexpr int LOADS   = 2 * PER; // loads per wavefront per tile
+
+    const int tid    = threadIdx.x;
+    const int lane   = tid & (WAVE - 1);
+    const int w      = tid / WAVE;
+    const int wm     = w % 2;
+    const int wn     = w / 2;
+    const int ktiles = kend > kbeg ? (kend - kbeg) / BK : 0;
+
+    const typename OA::Lane la = OA::lane_state(w, lane, m0, a.M);
+    const typename OB::Lane lb = OB::lane_state(w, lane, n0, a.N);
+
+    const double* Ap = a.A.get(b);
+    const double* Bp = a.B.get(b);
+    double*       Cp = a.C.get(b);
+
+    // tiles past the last one re-read the last tile (in bounds) into a
+    // stage nobody reads any more, so every step waits the same way
+    auto issue = [&](int kt, int st) {
+        const int64_t k0 = kbeg + int64_t(min(kt, ktiles - 1)) * BK;
+#pragma unroll
+        for(int g = 0; g < PER; ++g)
+            dma16(OA::src(Ap, la, g, w, k0, a.lda, a.M), &As[st][OA::dst(g, w)]);
+#pragma unroll
+        for(int g = 0; g < PER; ++g)
+            dma16(OB::src(Bp, lb, g, w, k0, a.ldb, a.N), &Bs[st][OB::dst(g, w)]);
+    };
+
+    d4_t acc[4][4];
+#pragma unroll
+    for(int i = 0; i < 4; ++i)
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+            acc[i][j] = d4_t{0, 0, 0, 0};
+
+    // A K step of the diagonal block: the triangular operand's stage is
+    // cleared outside its triangle (whatever the other triangle holds never
+    // enters) and set to one on a unit diagonal, 4 elements per thread,
+    // before any wavefront reads it.
+    auto mask_stage = [&](int st, int k0) {
+        double* stage = TRI_A ? As[st] : Bs[st];
+        // opaque copy of the thread index: keeps the compiler from holding
+        // these addresses over the whole K loop (registers the kernel lacks)
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        const int x    = t & 127;
+        const int unit = a.tri & GT_UNIT;
+#pragma unroll
+        for(int q = 0; q < 4; ++q)
+        {
+            const int k = (t >> 7) * 4 + q;
+            const int e = k0 + k - ((TRI_A ? m0 : n0) + x); // k - row (A) / column (B)
+            const int o = TRI_A ? OA::at(k, x) : OB::at(k, x);
+            if(TO_END ? e > 0 : e < 0)
+                stage[o] = 0.0;
+            else if(unit && e == 0)
+                stage[o] = 1.0;
+        }
+    };
+    auto compute = [&](int st) {
+        const double* as = As[st];
+        const double* bs = Bs[st];
+#pragma unroll
+        for(int k4 = 0; k4 < BK / 4; ++k4)
+        {
+            const int kr = k4 * 4 + lane / 16;
+            double    af[4], bf[4];
+#pragma unroll
+            for(int i = 0; i < 4; ++i)
+                af[i] = as[OA::at(kr, wm * 64 + i * 16 + (lane & 15))];
+#pragma unroll
+            for(int j = 0; j < 4; ++j)
+                bf[j] = bs[OB::at(kr, wn * 64 + j * 16 + (lane & 15))];
+#pragma unroll
+            for(int i = 0; i < 4; ++i)
+#pragma unroll
+                for(int j = 0; j < 4; ++j)
+                    acc[i][j] = Mfma<double>::run(bf[j], af[i], acc[i][j]);
+        }
+    };
+    // one K step: tiles kt .. kt + NS - 2 are in flight (LOADS each), so
+    // waiting down to (NS - 2) * LOADS retires tile kt; the barrier
+    // publishes it to every wavefront and retires every read of the stage
+    // tile kt + NS - 1 reuses
+    auto step = [&](int kt, int st) {
+        wait_vmcnt<(NS - 2) * LOADS>();
+        barrier_no_fence();
+        issue(kt + NS - 1, (st + NS - 1) % NS);
+        if constexpr(TRI != 0)
+        {
+            const int k0 = kbeg + kt * BK;
+            if(k0 >= dlo && k0 < dhi)
+            {
+                mask_stage(st, k0);
+                barrier_no_fence();
+            }
+        }
+        compute(st);
+    };
+
+    if(ktiles > 0)
+    {
+#pragma unroll
+        for(int t = 0; t < NS - 1; ++t)
+            issue(t, t);
+        for(int kt = 0; kt < ktiles; kt += NS)
+        {
+#pragma unroll
+            for(int t = 0; t < NS; ++t)
+                if(t == 0 || kt + t < ktiles)
+                    step(kt + t, t);
+        }
+        wait_vmcnt<0>(); // the tiles past the end
+    }
+
+    if(part)
+    {
+#pragma unroll
+        for(int i = 0; i < 4; ++i)
+#pragma unroll
+            for(int j = 0; j < 4; ++j)
+#pragma unroll
+                for(int r = 0; r < 4; ++r)
+                    part[((i * 4 + j) * 4 + r) * 256 + tid] = acc[i][j][r];
+        __syncthreads(); // LDS reuse by the next job
+        return;
+    }
+
+    // Pieces of a tile add into C one after another, in piece order, so the
+    // sum is the same whatever order they finish in: piece p waits until p
+    // pieces have finished (a bounded poll by one thread, then every
+    // wavefront acquires), piece 0 applies beta.  Earlier pieces have lower
+    // workgroup ids and were dispatched first, so every wait ends.
+    if constexpr(TRI != 0)
+    {
+        if(parts > 1 && piece > 0)
+            split_wait(cnt, unsigned(piece) + a.need_bias, a.spin_limit, a.split_err);
+    }
+
+    // epilogue: as gemm_kernel's
+#pragma unroll
+    for(int i = 0; i < 4; ++i)
+    {
+        const int row = m0 + wm * 64 + i * 16 + (lane & 15);
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+#pragma unroll
+            for(int r = 0; r < 4; ++r)
+            {
+                const int col = n0 + wn * 64 + j * 16 + Mfma<double>::row(lane, r);
+                if(row < a.M && col < a.N)
+                {
+                    double& cref = Cp[row + int64_t(col) * a.ldc];
+                    double  v    = a.alpha * acc[i][j][r];
+                    if(piece > 0)
+                        v = cref + v;
+                    else if(a.beta != 0.0)
+                        v = fma(a.beta, cref, v);
+                    cref = v;
+                }
+            }
+    }
+    if constexpr(TRI != 0)
+    {
+        if(parts > 1)
+            split_signal(cnt, unsigned(parts));
+    }
+    __syncthreads(); // LDS reuse by the next job
+}
+
+// DMA GEMM, one workgroup per 128 x 128 tile (and instance, in y).  With a
+// triangular operand (TRI != 0) a tile takes the K range in which the
+// operand is non-zero for its rows (A) or columns (B), up to and including
+// their diagonal block; with a split (kc > 0) that range is cut into
+// P = ceil(len / kc) near-equal pieces, whole K steps each, piece p of tile t
+// being workgroup t + p * tiles (none past P).
 template <int BK, int NS, int OPA, int OPB, int TRI>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     gemm_dma_kernel(const GemmArgsT<double, double, double> a)
 {
-    constexpr bool TRI_A  = TRI == GT_A_LOWER || TRI == GT_A_UPPER;
-    constexpr bool TO_END = TRI == GT_A_LOWER || TRI == GT_B_UPPER; // range ends at the diagonal
     static_assert((BK == 8 || BK == 16) && NS >= 2 && NS <= 3, "configuration");
-    using OA             = DmaOperand<BK, OPA == 0>;
-    using OB             = DmaOperand<BK, OPB == 1>;
-    constexpr int PER    = BK / 4;  // load instructions per operand per wavefront
-    constexpr int LOADS  = 2 * PER; // loads per wavefront per tile
-    __shared__ __align__(16) double As[NS][OA::ELS];
-    __shared__ __align__(16) double Bs[NS][OB::ELS];
+    constexpr bool TRI_A  = TRI == GT_A_LOWER || TRI == GT_A_UPPER;
+    constexpr bool TO_END = TRI == GT_A_LOWER || TRI == GT_B_UPPER;
+    __shared__ __align__(16) double As[NS][DmaOperand<BK, OPA == 0>::ELS];
+    __shared__ __align__(16) double Bs[NS][DmaOperand<BK, OPB == 1>::ELS];
 
     if(a.skip_if_zero && *a.skip_if_zero == 0.0)
         return;
-
-    const int tid  = threadIdx.x;
-    const int lane = tid & (WAVE - 1);
-    const int w    = tid / WAVE;
-    const int wm   = w % 2;
-    const int wn   = w / 2;
 
     // tile index: the same order as gemm_kernel -- XCD-contiguous runs of
     // row-panel groups, except with a triangular operand, whose row (column)
     // panels carry unequal work: contiguous runs would give one XCD all the
     // long panels (1.9x the mean at 16384^2), so dispatch order is kept there
-    // (round-robin over the XCDs).  A split launch numbers its workgroups
-    // piece-major: piece p of tile t is workgroup t + p * tiles.
+    // (round-robin over the XCDs)
     const int total = a.tiles_m * a.tiles_n;
     int       bid   = blockIdx.x;
     int       piece = 0;
@@ -447,19 +633,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         const int x = bid % 8, l = bid / 8;
         bid         = x * q + min(x, r) + l;
     }
-    const int     tile   = bid;
-    constexpr int GROUP  = 8;
-    const int     grp    = bid / (GROUP * a.tiles_n);
-    const int     fm     = grp * GROUP;
-    const int     gsz    = min(a.tiles_m - fm, GROUP);
-    const int     m0     = (fm + (bid % gsz)) * 128;
-    const int     n0     = ((bid % (GROUP * a.tiles_n)) / gsz) * 128;
-    // A triangular operand (square, K x K): this tile takes the K range in
-    // which the operand is non-zero for its rows (A) or columns (B), up to and
-    // including their diagonal block [dlo, dhi).  With a split (kc > 0) the
-    // range is cut into P = ceil(len / kc) near-equal pieces, whole K steps
-    // each, and this workgroup takes piece `piece` (none past P).
-    int kbeg = 0, kend = a.K, parts = 1, dlo = 0, dhi = 0;
+    const int     tile  = bid;
+    constexpr int GROUP = 8;
+    const int     grp   = bid / (GROUP * a.tiles_n);
+    const int     fm    = grp * GROUP;
+    const int     gsz   = min(a.tiles_m - fm, GROUP);
+    const int     m0    = (fm + (bid % gsz)) * 128;
+    const int     n0    = ((bid % (GROUP * a.tiles_n)) / gsz) * 128;
+    int           kbeg = 0, kend = a.K, parts = 1, dlo = 0, dhi = 0;
     if constexpr(TRI != 0)
     {
         dlo = TRI_A ? m0 : n0;
@@ -471,171 +652,167 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         const int len = max(kend - kbeg, 0);
         if(a.kc > 0 && len > a.kc)
         {
-            parts         = (len + a.kc - 1) / a.kc;
-            const int pl  = ((len + parts - 1) / parts + BK - 1) / BK * BK;
-            kbeg          = min(kend, kbeg + piece * pl);
-            kend          = min(kend, kbeg + pl);
+            parts        = (len + a.kc - 1) / a.kc;
+            const int pl = ((len + parts - 1) / parts + BK - 1) / BK * BK;
+            kbeg         = min(kend, kbeg + piece * pl);
+            kend         = min(kend, kbeg + pl);
         }
         if(piece >= parts)
             return;
     }
-    const int ktiles = kend > kbeg ? (kend - kbeg) / BK : 0;
-
-    const typename OA::Lane la = OA::lane_state(w, lane, m0, a.M);
-    const typename OB::Lane lb = OB::lane_state(w, lane, n0, a.N);
-
     for(int b = blockIdx.y; b < a.batch; b += gridDim.y)
+        dma_job<BK, NS, OPA, OPB, TRI>(
+            a, As, Bs, b, m0, n0, kbeg, kend, dlo, dhi, piece, parts,
+            parts > 1 ? a.split_cnt + (int64_t(b) * total + tile) : nullptr, nullptr);
+}
+
+// Balanced triangular products.  The tiles of every instance, ordered
+// (instance, panel, tile in panel) with a panel the row (A) or column (B)
+// block of the triangle, lay their K steps end to end, N in all; workgroup w
+// of W (W <= N) takes steps [w N / W, (w + 1) N / W).  A tile wholly inside
+// one workgroup's range is written to C; the pieces of a tile cut between
+// workgroups go to the workspace, two slots of 128 x 128 per workgroup (its
+// first tile, its last tile), and gemm_dma_reduce_kernel adds them in
+// workgroup order.  Nothing waits and nothing is shared within the launch.
+struct BalancedSplit
+{
+    int     panels, other, total;
+    int64_t per_inst, n_all, W;
+
+    template <int TRI>
+    __device__ __forceinline__ static BalancedSplit of(const GemmArgsT<double, double, double>& a)
     {
-        const double* Ap = a.A.get(b);
-        const double* Bp = a.B.get(b);
-        double*       Cp = a.C.get(b);
+        constexpr bool TRI_A = TRI == GT_A_LOWER || TRI == GT_A_UPPER;
+        BalancedSplit  z{};
+        z.panels = TRI_A ? a.tiles_m : a.tiles_n;
+        z.other  = TRI_A ? a.tiles_n : a.tiles_m;
+        z.total  = a.tiles_m * a.tiles_n;
+        for(int q = 0; q < z.panels; ++q)
+            z.per_inst += int64_t(z.other) * z.steps<TRI>(a, q);
+        z.n_all = z.per_inst * a.batch;
+        z.W     = a.split_w;
+        return z;
+    }
+    // K steps (of 8) of the tiles in panel q
+    template <int TRI>
+    __device__ __forceinline__ static int steps(const GemmArgsT<double, double, double>& a, int q)
+    {
+        constexpr bool TO_END = TRI == GT_A_LOWER || TRI == GT_B_UPPER;
+        const int      lo = q * 128, hi = min(a.K, lo + 128);
+        return (TO_END ? hi : a.K - lo) / 8;
+    }
+    __device__ __forceinline__ int64_t first(int64_t w) const { return w * n_all / W; }
+    // the workgroup whose range holds step x
+    __device__ __forceinline__ int64_t owner(int64_t x) const
+    {
+        return ((x + 1) * W - 1) / n_all;
+    }
+};
 
-        // tiles past the last one re-read the last tile (in bounds) into a
-        // stage nobody reads any more, so every step waits the same way
-        auto issue = [&](int kt, int st) {
-            const int64_t k0 = (TRI != 0 ? kbeg : 0) + int64_t(min(kt, ktiles - 1)) * BK;
-#pragma unroll
-            for(int g = 0; g < PER; ++g)
-                dma16(OA::src(Ap, la, g, w, k0, a.lda, a.M), &As[st][OA::dst(g, w)]);
-#pragma unroll
-            for(int g = 0; g < PER; ++g)
-                dma16(OB::src(Bp, lb, g, w, k0, a.ldb, a.N), &Bs[st][OB::dst(g, w)]);
-        };
+template <int OPA, int OPB, int TRI>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    gemm_dma_balanced_kernel(const GemmArgsT<double, double, double> a)
+{
+    constexpr int  BK     = 8;
+    constexpr int  NS     = 3;
+    constexpr bool TRI_A  = TRI == GT_A_LOWER || TRI == GT_A_UPPER;
+    constexpr bool TO_END = TRI == GT_A_LOWER || TRI == GT_B_UPPER;
+    static_assert(TRI != 0, "triangular operands only");
+    __shared__ __align__(16) double As[NS][DmaOperand<BK, OPA == 0>::ELS];
+    __shared__ __align__(16) double Bs[NS][DmaOperand<BK, OPB == 1>::ELS];
 
-        d4_t acc[4][4];
-#pragma unroll
-        for(int i = 0; i < 4; ++i)
-#pragma unroll
-            for(int j = 0; j < 4; ++j)
-                acc[i][j] = d4_t{0, 0, 0, 0};
-
-        // A K step of the diagonal block: the triangular operand's stage is
-        // cleared outside its triangle (whatever the other triangle holds
-        // never enters) and set to one on a unit diagonal, 4 elements per
-        // thread, before any wavefront reads it.
-        auto mask_stage = [&](int st, int k0) {
-            double* stage = TRI_A ? As[st] : Bs[st];
-            // opaque copy of the thread index: keeps the compiler from
-            // holding these addresses over the whole K loop (registers the
-            // kernel lacks)
-            int t = tid;
-            asm volatile("" : "+v"(t));
-            const int x    = t & 127;
-            const int unit = a.tri & GT_UNIT;
-#pragma unroll
-            for(int q = 0; q < 4; ++q)
-            {
-                const int k = (t >> 7) * 4 + q;
-                const int e = k0 + k - ((TRI_A ? m0 : n0) + x); // k - row (A) / column (B)
-                const int o = TRI_A ? OA::at(k, x) : OB::at(k, x);
-                if(TO_END ? e > 0 : e < 0)
-                    stage[o] = 0.0;
-                else if(unit && e == 0)
-                    stage[o] = 1.0;
-            }
-        };
-        auto compute = [&](int st) {
-            const double* as = As[st];
-            const double* bs = Bs[st];
-#pragma unroll
-            for(int k4 = 0; k4 < BK / 4; ++k4)
-            {
-                const int kr = k4 * 4 + lane / 16;
-                double    af[4], bf[4];
-#pragma unroll
-                for(int i = 0; i < 4; ++i)
-                    af[i] = as[OA::at(kr, wm * 64 + i * 16 + (lane & 15))];
-#pragma unroll
-                for(int j = 0; j < 4; ++j)
-                    bf[j] = bs[OB::at(kr, wn * 64 + j * 16 + (lane & 15))];
-#pragma unroll
-                for(int i = 0; i < 4; ++i)
-#pragma unroll
-                    for(int j = 0; j < 4; ++j)
-                        acc[i][j] = Mfma<double>::run(bf[j], af[i], acc[i][j]);
-            }
-        };
-        // one K step: tiles kt .. kt + NS - 2 are in flight (LOADS each), so
-        // waiting down to (NS - 2) * LOADS retires tile kt; the barrier
-        // publishes it to every wavefront and retires every read of the stage
-        // tile kt + NS - 1 reuses
-        auto step = [&](int kt, int st) {
-            wait_vmcnt<(NS - 2) * LOADS>();
-            barrier_no_fence();
-            issue(kt + NS - 1, (st + NS - 1) % NS);
-            if constexpr(TRI != 0)
-            {
-                const int k0 = kbeg + kt * BK;
-                if(k0 >= dlo && k0 < dhi)
-                {
-                    mask_stage(st, k0);
-                    barrier_no_fence();
-                }
-            }
-            compute(st);
-        };
-
-        if(ktiles > 0)
+    const BalancedSplit z  = BalancedSplit::of<TRI>(a);
+    const int64_t       wg = blockIdx.x;
+    const int64_t       s0 = z.first(wg);
+    int64_t             e  = z.first(wg + 1);
+    while(e > s0)
+    {
+        // the tile holding step e - 1: instance b, panel q, tile i of the panel
+        const int64_t x  = e - 1;
+        const int     b  = int(x / z.per_inst);
+        int64_t       r  = x - int64_t(b) * z.per_inst;
+        int           q  = 0;
+        int           ns = BalancedSplit::steps<TRI>(a, 0);
+        while(r >= int64_t(z.other) * ns)
         {
-#pragma unroll
-            for(int t = 0; t < NS - 1; ++t)
-                issue(t, t);
-            for(int kt = 0; kt < ktiles; kt += NS)
-            {
-#pragma unroll
-                for(int t = 0; t < NS; ++t)
-                    if(t == 0 || kt + t < ktiles)
-                        step(kt + t, t);
-            }
-            wait_vmcnt<0>(); // the tiles past the end
+            r -= int64_t(z.other) * ns;
+            ++q;
+            ns = BalancedSplit::steps<TRI>(a, q);
         }
+        const int     i     = int(r / ns);
+        const int64_t ts    = x - r % ns; // the tile's first step
+        const int64_t js    = max(s0, ts);
+        const bool    whole = js == ts && e == ts + ns;
+        const int     dlo   = q * 128;
+        const int     kb    = TO_END ? 0 : dlo;
+        double*       part  = whole ? nullptr
+                                    : a.split_ws + (2 * wg + (js == s0 ? 0 : 1)) * int64_t(16384);
+        dma_job<BK, NS, OPA, OPB, TRI>(a, As, Bs, b, (TRI_A ? q : i) * 128, (TRI_A ? i : q) * 128,
+                                       kb + int(js - ts) * BK, kb + int(e - ts) * BK, dlo,
+                                       min(a.K, dlo + 128), 0, 1, nullptr, part);
+        e = js;
+    }
+}
 
-        // Split pieces of a tile add into C one after another, in piece
-        // order, so the sum is the same whatever order they finish in: piece
-        // p waits until p pieces have finished (a bounded poll by one thread,
-        // then every wavefront acquires), piece 0 applies beta.  Earlier
-        // pieces have lower workgroup ids and were dispatched first, so every
-        // wait ends.
-        unsigned* cnt = nullptr;
-        if constexpr(TRI != 0)
-        {
-            if(parts > 1)
-            {
-                cnt = a.split_cnt + (int64_t(b) * total + tile);
-                if(piece > 0)
-                    split_wait(cnt, unsigned(piece) + a.need_bias, a.spin_limit, a.split_err);
-            }
-        }
+// The split tiles of a balanced product: C = alpha * (the pieces' sums, in
+// workgroup order) + beta * C.  One workgroup per tile and instance; tiles
+// that one workgroup computed whole return at once.
+template <int TRI>
+__global__ void __launch_bounds__(256)
+    gemm_dma_reduce_kernel(const GemmArgsT<double, double, double> a)
+{
+    constexpr bool      TRI_A = TRI == GT_A_LOWER || TRI == GT_A_UPPER;
+    const BalancedSplit z     = BalancedSplit::of<TRI>(a);
+    const int           b     = blockIdx.x / z.total;
+    const int           t     = blockIdx.x - b * z.total;
+    const int           q     = t / z.other;
+    const int           i     = t - q * z.other;
+    int64_t             ts    = int64_t(b) * z.per_inst;
+    for(int p = 0; p < q; ++p)
+        ts += int64_t(z.other) * BalancedSplit::steps<TRI>(a, p);
+    const int     ns = BalancedSplit::steps<TRI>(a, q);
+    ts += int64_t(i) * ns;
+    const int64_t w0 = z.owner(ts), wl = z.owner(ts + ns - 1);
+    if(w0 == wl)
+        return;
 
-        // epilogue: as gemm_kernel's
+    const int tid  = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int w    = tid / WAVE;
+    const int m0   = (TRI_A ? q : i) * 128;
+    const int n0   = (TRI_A ? i : q) * 128;
+    double*   Cp   = a.C.get(b);
+    // the piece of workgroup g sits in its first-tile slot unless g's range
+    // starts before this tile; a piece's 64 values load together
+    auto piece = [&](int64_t g) {
+        return a.split_ws + (2 * g + (z.first(g) >= ts ? 0 : 1)) * int64_t(16384) + tid;
+    };
+    double        sum[64];
+    const double* p0 = piece(w0);
 #pragma unroll
-        for(int i = 0; i < 4; ++i)
+    for(int v = 0; v < 64; ++v)
+        sum[v] = p0[v * 256];
+    for(int64_t g = w0 + 1; g <= wl; ++g)
+    {
+        const double* pg = piece(g);
+#pragma unroll
+        for(int v = 0; v < 64; ++v)
+            sum[v] += pg[v * 256];
+    }
+#pragma unroll
+    for(int v = 0; v < 64; ++v)
+    {
+        const int i4 = v / 16, j4 = (v / 4) % 4, r = v % 4;
+        const int row = m0 + (w % 2) * 64 + i4 * 16 + (lane & 15);
+        const int col = n0 + (w / 2) * 64 + j4 * 16 + Mfma<double>::row(lane, r);
+        if(row < a.M && col < a.N)
         {
-            const int row = m0 + wm * 64 + i * 16 + (lane & 15);
-#pragma unroll
-            for(int j = 0; j < 4; ++j)
-#pragma unroll
-                for(int r = 0; r < 4; ++r)
-                {
-                    const int col = n0 + wn * 64 + j * 16 + Mfma<double>::row(lane, r);
-                    if(row < a.M && col < a.N)
-                    {
-                        double& cref = Cp[row + int64_t(col) * a.ldc];
-                        double  v    = a.alpha * acc[i][j][r];
-                        if(piece > 0)
-                            v = cref + v;
-                        else if(a.beta != 0.0)
-                            v = fma(a.beta, cref, v);
-                        cref = v;
-                    }
-                }
+            double& cref = Cp[row + int64_t(col) * a.ldc];
+            double  out  = a.alpha * sum[v];
+            if(a.beta != 0.0)
+                out = fma(a.beta, cref, out);
+            cref = out;
         }
-        if constexpr(TRI != 0)
-        {
-            if(parts > 1)
-                split_signal(cnt, unsigned(parts));
-        }
-        __syncthreads(); // LDS reuse across batch iterations
     }
 }
 
@@ -1098,6 +1275,105 @@ static rocblas_status launch_gemm_dma_split(rocblas_handle h, const GemmArgsT<do
     return launch_status();
 }
 
+// Shortest K of a triangular product balanced over the workgroups; shorter
+// ones run on the masked register-staged kernels, which measured as fast or
+// faster there (K = 512, 4 groups of 4 x 4 tiles: 83 vs 92 us for the two
+// products of a level, the reduce passes included).  ROCBLAS_AMD_BALANCED_MIN_K
+// overrides it for A/B comparisons.
+static int balanced_min_k()
+{
+    static const int v = [] {
+        const char* e = std::getenv("ROCBLAS_AMD_BALANCED_MIN_K");
+        return e ? std::max(128, std::atoi(e)) : 1024;
+    }();
+    return v;
+}
+
+// Workgroups of a balanced launch of a triangular product by shape (0: the
+// shape does not qualify): every K step of its tiles, over
+// min(2 x CUs, steps / the shortest piece) workgroups.
+static int64_t balanced_shape_workgroups(rocblas_handle h, int M, int N, int K, int tri, int batch)
+{
+    const int kind = tri & (GT_A_LOWER | GT_A_UPPER | GT_B_LOWER | GT_B_UPPER);
+    if(!kind || !gemm_dma_mode() || !gemm_split_min() || M < 128 || N < 128
+       || K < balanced_min_k() || K % 8)
+        return 0;
+    const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
+    if(tm * tn * batch >= 2 * int64_t(h->num_cus) || tm * tn * batch > kSplitCounters)
+        return 0;
+    const bool onA   = kind == GT_A_LOWER || kind == GT_A_UPPER;
+    const bool toEnd = kind == GT_A_LOWER || kind == GT_B_UPPER;
+    int64_t    steps = 0;
+    for(int64_t q = 0; q < (onA ? tm : tn); ++q)
+    {
+        const int64_t lo = q * 128, hi = std::min<int64_t>(K, lo + 128);
+        steps += (onA ? tn : tm) * ((toEnd ? hi : K - lo) / 8);
+    }
+    steps *= batch;
+    const int64_t W = std::min<int64_t>(2 * int64_t(h->num_cus),
+                                        std::max<int64_t>(1, steps / (gemm_split_min() / 8)));
+    return W <= steps ? W : 0;
+}
+
+// The balanced launch of `a`, if its shape qualifies, the caller allows a
+// split and the workspace holds its 2 W partial tiles.
+static int64_t balanced_workgroups(rocblas_handle h, const GemmArgsT<double, double, double>& a)
+{
+    if(!a.tri || !a.split_ok || !a.split_ws || !gemm_dma_eligible(a, 8))
+        return 0;
+    const int64_t W = balanced_shape_workgroups(h, a.M, a.N, a.K, a.tri, a.batch);
+    return 2 * W <= a.split_ws_tiles ? W : 0;
+}
+
+size_t gemm_tri_split_bytes(rocblas_handle h, int M, int N, int K, int tri, int batch)
+{
+    return size_t(2 * balanced_shape_workgroups(h, M, N, K, tri, batch)) * 128 * 128
+           * sizeof(double);
+}
+
+template <int OPA, int OPB, int TRI>
+static void launch_balanced_ops(rocblas_handle h, const GemmArgsT<double, double, double>& d)
+{
+    hipLaunchKernelGGL((gemm_dma_balanced_kernel<OPA, OPB, TRI>), dim3(unsigned(d.split_w)),
+                       dim3(256), 0, h->stream, d);
+    hipLaunchKernelGGL((gemm_dma_reduce_kernel<TRI>),
+                       dim3(unsigned(int64_t(d.tiles_m) * d.tiles_n * d.batch)), dim3(256), 0,
+                       h->stream, d);
+}
+
+template <int TRI>
+static void launch_balanced_kind(rocblas_handle h, const GemmArgsT<double, double, double>& d)
+{
+    if(d.opA == 0 && d.opB == 0)
+        launch_balanced_ops<0, 0, TRI>(h, d);
+    else if(d.opA == 1)
+        launch_balanced_ops<1, 0, TRI>(h, d);
+    else
+        launch_balanced_ops<0, 1, TRI>(h, d);
+}
+
+// A triangular product with few tiles, its K steps balanced over W
+// workgroups (gemm_dma_balanced_kernel) and the split tiles summed in
+// workgroup order (gemm_dma_reduce_kernel): the result depends only on the
+// shapes and the CU count.
+static rocblas_status launch_gemm_dma_balanced(rocblas_handle h,
+                                               const GemmArgsT<double, double, double>& a,
+                                               int64_t W)
+{
+    GemmArgsT<double, double, double> d = a;
+    d.tiles_m                           = (a.M + 127) / 128;
+    d.tiles_n                           = (a.N + 127) / 128;
+    d.split_w                           = int(W);
+    switch(a.tri & (GT_A_LOWER | GT_A_UPPER | GT_B_LOWER | GT_B_UPPER))
+    {
+    case GT_A_LOWER: launch_balanced_kind<GT_A_LOWER>(h, d); break;
+    case GT_A_UPPER: launch_balanced_kind<GT_A_UPPER>(h, d); break;
+    case GT_B_LOWER: launch_balanced_kind<GT_B_LOWER>(h, d); break;
+    default: launch_balanced_kind<GT_B_UPPER>(h, d); break;
+    }
+    return launch_status();
+}
+
 // Pick the largest tile that still gives every CU work.
 template <typename TA, typename TB, typename TC>
 static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, TC>& a)
@@ -1114,14 +1390,17 @@ static rocblas_status launch_gemm_sel(rocblas_handle h, const GemmArgsT<TA, TB, 
     if(a.M <= 32)
         return tiles(32, 128) >= cus ? launch_gemm_cfg<TA, TB, TC, 32, 128, 16, 1, 4>(h, a)
                                      : launch_gemm_cfg<TA, TB, TC, 32, 32, 16, 2, 2>(h, a);
-    // one triangular product whose tiles fill between a quarter and all of
-    // the chip's workgroup slots: the DMA kernel with the K ranges split,
-    // where the caller allows it (fewer tiles: the pieces would be too short
-    // for the DMA kernel's per-step latency, and the masked kernels below run
-    // faster)
+    // triangular products whose tiles alone would leave workgroup slots idle,
+    // where the caller allows a split: with a workspace, the K steps balanced
+    // over the workgroups; without one, a single product with 128-511 tiles
+    // on the DMA kernel with its tiles' K ranges split (fewer tiles: the
+    // pieces would be too short for the DMA kernel's per-step latency, and
+    // the masked kernels below run faster)
     if constexpr(std::is_same_v<TA, double> && std::is_same_v<TB, double>
                  && std::is_same_v<TC, double>)
     {
+        if(const int64_t W = balanced_workgroups(h, a))
+            return launch_gemm_dma_balanced(h, a, W);
         if(a.tri && a.split_ok && a.batch == 1 && gemm_dma_mode() && tiles(128, 128) * 2 >= cus
            && tiles(128, 128) < 2 * cus && gemm_dma_eligible(a, 8))
             return launch_gemm_dma_split(h, a);

@@ -179,7 +179,9 @@ rocblas_status trsm_launch(rocblas_handle h, const TrsmProblem<T>& p);
 
 // invA := op(A)^-1 for batch_count n x n triangles (no argument checks).  With
 // scratch (n * n * batch_count elements) A is copied there first, which makes
-// invA == A (in place) safe.
+// invA == A (in place) safe.  split_ws (split_bytes, from trtri_split_bytes)
+// lets the doubling products with few tiles balance their K steps over the
+// workgroups; nullptr runs them without.
 template <typename T>
 rocblas_status trtri_core(rocblas_handle    handle,
                           rocblas_fill      uplo,
@@ -190,7 +192,15 @@ rocblas_status trtri_core(rocblas_handle    handle,
                           BatchPtr<T>       invA,
                           int               ldinvA,
                           int               batch_count,
-                          T*                scratch);
+                          T*                scratch,
+                          void*             split_ws,
+                          size_t            split_bytes);
+
+// Workspace bytes trtri_core's balanced products use for an order-n
+// inversion of `batch` instances (groups_batched: the single-instance form
+// batches a level's equal groups, which pointer arrays do not).
+template <typename T>
+size_t trtri_split_bytes(rocblas_handle h, int n, int batch, bool groups_batched);
 
 template <typename T>
 rocblas_status zero_matrix_launch(rocblas_handle h, int m, int n, BatchPtr<T> B, int ldb,

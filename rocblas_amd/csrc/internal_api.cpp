@@ -436,11 +436,20 @@ try
                           == static_cast<const void*>(invA + offset_invA);
     if(in_place && !w_C_tmp)
         return rocblas_status_invalid_pointer;
+    // the split products' partial tiles, from the handle's workspace when it
+    // has room (without, the products run unsplit)
+    const size_t split_bytes
+        = std::max({trtri_split_bytes<T>(handle, n, batch_count, true),
+                    trtri_split_bytes<T>(handle, n, sub_batch_count, true),
+                    trtri_split_bytes<T>(handle, n, 1, true)});
+    WorkspaceLease lease(handle, split_bytes);
+    void* const    sw = lease.ptr;
+    const size_t   sb = lease.ptr ? split_bytes : 0;
     if(sub_batch_count == 1)
         return trtri_core<T>(handle, uplo, diag, n,
                              BatchPtr<const T>{nullptr, A, stride_A, offset_A}, lda,
                              BatchPtr<T>{nullptr, invA, stride_invA, offset_invA}, ldinvA,
-                             batch_count, in_place ? w_C_tmp : nullptr);
+                             batch_count, in_place ? w_C_tmp : nullptr, sw, sb);
     for(rocblas_int b = 0; b < batch_count; ++b)
     {
         const rocblas_stride oa = offset_A + b * stride_A, oi = offset_invA + b * stride_invA;
@@ -449,14 +458,14 @@ try
             st = trtri_core<T>(handle, uplo, diag, n,
                                BatchPtr<const T>{nullptr, A, sub_stride_A, oa}, lda,
                                BatchPtr<T>{nullptr, invA, sub_stride_invA, oi}, ldinvA,
-                               sub_batch_count, nullptr);
+                               sub_batch_count, nullptr, sw, sb);
         else
             // in place, one matrix at a time through the n x n scratch
             for(rocblas_int s = 0; s < sub_batch_count && st == rocblas_status_success; ++s)
                 st = trtri_core<T>(handle, uplo, diag, n,
                                    BatchPtr<const T>{nullptr, A, 0, oa + s * sub_stride_A}, lda,
                                    BatchPtr<T>{nullptr, invA, 0, oi + s * sub_stride_invA},
-                                   ldinvA, 1, w_C_tmp);
+                                   ldinvA, 1, w_C_tmp, sw, sb);
         if(st != rocblas_status_success)
             return st;
     }
@@ -495,13 +504,15 @@ try
     // the same offsets) is not supported in this form
     if(static_cast<const void*>(A) == static_cast<const void*>(invA) && offset_A == offset_invA)
         return rocblas_status_not_implemented;
+    const size_t   split_bytes = trtri_split_bytes<T>(handle, n, batch_count, false);
+    WorkspaceLease lease(handle, split_bytes);
     for(rocblas_int s = 0; s < sub_batch_count; ++s)
     {
         const rocblas_status st = trtri_core<T>(
             handle, uplo, diag, n,
             BatchPtr<const T>{A, nullptr, 0, offset_A + s * sub_stride_A}, lda,
             BatchPtr<T>{invA, nullptr, 0, offset_invA + s * sub_stride_invA}, ldinvA, batch_count,
-            nullptr);
+            nullptr, lease.ptr, lease.ptr ? split_bytes : 0);
         if(st != rocblas_status_success)
             return st;
     }

@@ -13,9 +13,43 @@
 #include "device_common.hpp"
 
 #include <new>
+#include <optional>
+#include <type_traits>
 
 namespace rbamd
 {
+// Workspace the balanced split products of an order-n inversion use: the
+// largest need of any FP64 doubling product (trtri_core's levels; equal
+// groups as the batch when `groups_batched`), 0 for FP32.
+template <typename T>
+size_t trtri_split_bytes(rocblas_handle h, int n, int batch, bool groups_batched)
+{
+    if constexpr(!std::is_same_v<T, double>)
+        return 0;
+    else
+    {
+        size_t need = 0;
+        for(int64_t s = (n <= 16 ? 16 : kTrtriNB); s < n; s *= 2)
+        {
+            const int64_t full = n / (2 * s), n2last = n - full * 2 * s - s;
+            if(full > 0)
+            {
+                const int gb = batch == 1 && full > 1 && groups_batched ? int(full) : batch;
+                for(int tri : {GT_A_LOWER, GT_B_LOWER})
+                    need = std::max(need, gemm_tri_split_bytes(h, int(s), int(s), int(s), tri, gb));
+            }
+            if(n2last > 0)
+            {
+                need = std::max(need, gemm_tri_split_bytes(h, int(n2last), int(s), int(n2last),
+                                                           GT_A_LOWER, batch));
+                need = std::max(need, gemm_tri_split_bytes(h, int(s), int(n2last), int(s),
+                                                           GT_A_UPPER, batch));
+            }
+        }
+        return need;
+    }
+}
+
 namespace
 {
 
@@ -36,8 +70,12 @@ rocblas_status trtri_entry(rocblas_handle    handle,
     if(!handle)
         return rocblas_status_invalid_handle;
 
-    const size_t ws_bytes
+    const size_t copy_bytes
         = in_place && n > 0 && batch_count > 0 ? size_t(batch_count) * n * n * sizeof(T) : 0;
+    const size_t split_bytes = n > 0 && batch_count > 0 ? trtri_split_bytes<T>(
+                                   handle, n, batch_count, !A.arr && !invA.arr)
+                                                        : 0;
+    const size_t ws_bytes = roundup_ws(copy_bytes) + split_bytes;
     if(handle->size_query)
     {
         const size_t need = roundup_ws(ws_bytes);
@@ -58,11 +96,24 @@ rocblas_status trtri_entry(rocblas_handle    handle,
     if(A_null || invA_null)
         return rocblas_status_invalid_pointer;
 
-    WorkspaceLease lease(handle, ws_bytes);
-    if(in_place && !lease.ptr)
+    // the split products' partial tiles after the in-place copy; without
+    // room for them (a user-managed workspace too small, a captured stream)
+    // the products run without (other last bits)
+    std::optional<WorkspaceLease> lease;
+    lease.emplace(handle, ws_bytes);
+    bool has_split = split_bytes && lease->ptr;
+    if(!lease->ptr && split_bytes)
+    {
+        lease.reset();
+        lease.emplace(handle, copy_bytes);
+    }
+    if(in_place && !lease->ptr)
         return rocblas_status_memory_error;
+    char* const ws = static_cast<char*>(lease->ptr);
     return trtri_core<T>(handle, uplo, diag, n, A, lda, invA, ldinvA, batch_count,
-                         in_place ? static_cast<T*>(lease.ptr) : nullptr);
+                         in_place ? reinterpret_cast<T*>(ws) : nullptr,
+                         has_split ? ws + roundup_ws(copy_bytes) : nullptr,
+                         has_split ? split_bytes : 0);
 }
 
 } // namespace
@@ -90,7 +141,9 @@ rocblas_status trtri_double_step(rocblas_handle    h,
                                  int               lda,
                                  BatchPtr<T>       X,
                                  int               ldx,
-                                 int               batch)
+                                 int               batch,
+                                 void*             split_ws,
+                                 size_t            split_bytes)
 {
     // operand views at element offset `off` of each instance (or of each
     // group, when the groups are the batch)
@@ -118,6 +171,8 @@ rocblas_status trtri_double_step(rocblas_handle    h,
     g.beta         = T(0);
     g.skip_if_zero = nullptr;
     g.split_ok     = 1; // products with few tiles may split their K ranges
+    g.split_ws       = static_cast<double*>(split_ws);
+    g.split_ws_tiles = int64_t(split_bytes / (128 * 128 * sizeof(double)));
     g.batch        = multi ? groups : batch;
     g.lda          = lda;
     g.ldb          = ldx;
@@ -210,7 +265,9 @@ rocblas_status trtri_core(rocblas_handle    handle,
                           BatchPtr<T>       invA,
                           int               ldinvA,
                           int               batch_count,
-                          T*                scratch)
+                          T*                scratch,
+                          void*             split_ws,
+                          size_t            split_bytes)
 {
     // a split product of an earlier call gave up a wait: report it here
     if(rocblas_status st = take_async_error(handle); st != rocblas_status_success)
@@ -252,18 +309,20 @@ rocblas_status trtri_core(rocblas_handle    handle,
         {
             if(batch_count == 1 && full > 1 && !Asrc.arr && !invA.arr)
                 st = trtri_double_step<T>(handle, upper, 0, int(s), int(s), int(full), span, Asrc,
-                                          ldas, invA, ldinvA, 1);
+                                          ldas, invA, ldinvA, 1, split_ws, split_bytes);
             else
                 for(int64_t gi = 0; gi < full && st == rocblas_status_success; ++gi)
                     st = trtri_double_step<T>(handle, upper, int(gi * span), int(s), int(s), 1,
-                                              span, Asrc, ldas, invA, ldinvA, batch_count);
+                                              span, Asrc, ldas, invA, ldinvA, batch_count,
+                                              split_ws, split_bytes);
             if(st != rocblas_status_success)
                 return st;
         }
         if(n2last > 0)
         {
             st = trtri_double_step<T>(handle, upper, int(r_last), int(s), int(n2last), 1, span,
-                                      Asrc, ldas, invA, ldinvA, batch_count);
+                                      Asrc, ldas, invA, ldinvA, batch_count, split_ws,
+                                      split_bytes);
             if(st != rocblas_status_success)
                 return st;
         }
@@ -273,10 +332,12 @@ rocblas_status trtri_core(rocblas_handle    handle,
 
 template rocblas_status trtri_core<float>(rocblas_handle, rocblas_fill, rocblas_diagonal, int,
                                           BatchPtr<const float>, int, BatchPtr<float>, int, int,
-                                          float*);
+                                          float*, void*, size_t);
 template rocblas_status trtri_core<double>(rocblas_handle, rocblas_fill, rocblas_diagonal, int,
                                            BatchPtr<const double>, int, BatchPtr<double>, int,
-                                           int, double*);
+                                           int, double*, void*, size_t);
+template size_t trtri_split_bytes<float>(rocblas_handle, int, int, bool);
+template size_t trtri_split_bytes<double>(rocblas_handle, int, int, bool);
 
 } // namespace rbamd
 

@@ -142,8 +142,9 @@ def test_gemm_lds_dma_loads_are_set_up(tmp_path):
     blocks = re.findall(r"\.amdhsa_kernel (\S+)\n(.*?)\.end_amdhsa_kernel", asm, re.S)
     dma = [(n, b) for n, b in blocks if "gemm_dma" in n]
     # FP64: 2 depths x 4 transpose forms dense, 3 forms for each of the 4
-    # kinds of triangular operand (8-deep only); FP32: 2 x 4
-    assert len(dma) == 28, len(dma)
+    # kinds of triangular operand (8-deep only), the same 12 as the balanced
+    # kernel and its 4 reduce kernels; FP32: 2 x 4
+    assert len(dma) == 44, len(dma)
     for n, b in dma:
         size = re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", b)
         assert size and int(size.group(1)) == 0, (n[:60], size and size.group(1))

@@ -296,17 +296,69 @@ def test_split_products_accuracy(h, uplo):
         torch.cuda.empty_cache()
 
 
+def _small_workspace_handle():
+    """A handle whose user-managed workspace (1 MiB) cannot hold the balanced
+    products' partial tiles, so trtri's top level at n = 4096 splits its tiles'
+    K ranges with in-order waits instead."""
+    ht = GpuHandle()
+    ws = torch.empty(1 << 17, dtype=torch.float64, device="cuda")
+    assert rb.lib().rocblas_set_workspace(ht.ptr, ws.data_ptr(), ws.numel() * 8) == S.success
+    return ht, ws
+
+
+def test_trtri_workspace_query_and_fallback(h):
+    """The size query asks for the balanced products' partial tiles (n = 4096:
+    2 x 512 tiles of 128 x 128 doubles with 256 CUs); with a workspace too small
+    for them the call still succeeds (the other split), exact on the bidiagonal
+    and accurate on a random triangle."""
+    n = 4096
+    sz = ctypes.c_size_t(0)
+    L = rb.lib()
+    assert L.rocblas_start_device_memory_size_query(h.ptr) == S.success
+    st = rb.trtri(h, "L", "N", n, 8, n, 16, n)
+    assert st in (S.size_unchanged, S.size_increased)
+    assert L.rocblas_stop_device_memory_size_query(h.ptr, ctypes.byref(sz)) == S.success
+    # the top level: 16 x 16 tiles whose K steps of 8 sum to 16 x 16 x 136
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    workgroups = min(2 * cus, 16 * 16 * 136 // 16)
+    assert sz.value >= 2 * workgroups * 128 * 128 * 8, sz.value
+    ht, ws = _small_workspace_handle()
+    try:
+        for uplo in "LU":
+            say("trtri small workspace", uplo)
+            dA = _bidiagonal(n, uplo, 1.0, 0.0)
+            dI = torch.full((n * n,), 5.0, dtype=torch.float64, device="cuda")
+            assert rb.trtri(ht, uplo, "N", n, dA.data_ptr(), n, dI.data_ptr(), n) == S.success
+            torch.cuda.synchronize()
+            ones = torch.ones(n, n, dtype=torch.float64, device="cuda")
+            assert torch.equal(dI, (ones.triu() if uplo == "L" else ones.tril()).reshape(-1))
+            dR = torch.empty(n * n, dtype=torch.float64, device="cuda")
+            assert rb.synth_tri(ht, uplo, "uniform", n, 11, dR.data_ptr(), n, scale=1.0 / n) \
+                == S.success
+            assert rb.trtri(ht, uplo, "N", n, dR.data_ptr(), n, dI.data_ptr(), n) == S.success
+            torch.cuda.synchronize()
+            T = (torch.tril if uplo == "L" else torch.triu)(dR.view(n, n).T)
+            err = float((dI.view(n, n).T @ T - torch.eye(n, dtype=torch.float64,
+                                                          device="cuda")).abs().max())
+            assert err <= 1e-12, err
+            assert ht.get_async_status() == S.success
+    finally:
+        ht.close()
+        del ws
+
+
 def test_split_wait_timeout_reported_and_healed(h):
-    """The split products' bounded waits, forced to give up through the
-    handle's test hook (every kernel still runs to its end): the handle reports
-    the error once, the next trtri call returns it without work, and after the
-    hook is cleared the tile counters are back at zero, so the result is the
-    exact one again."""
+    """The split products' bounded waits (the in-order form, taken when the
+    workspace cannot hold the balanced products' partial tiles), forced to
+    give up through the handle's test hook (every kernel still runs to its
+    end): the handle reports the error once, the next trtri call returns it
+    without work, and after the hook is cleared the tile counters are back at
+    zero, so the result is the exact one again."""
     n, uplo = 4096, "L"
     dA = _bidiagonal(n, uplo, 1.0, 0.0)
     ones = torch.ones(n, n, dtype=torch.float64, device="cuda")
     want = ones.triu().reshape(-1)
-    ht = GpuHandle()
+    ht, ws = _small_workspace_handle()
     try:
         assert ht.set_chain_test_timeout(1) == S.success
         dI = torch.zeros(n * n, dtype=torch.float64, device="cuda")
@@ -330,3 +382,4 @@ def test_split_wait_timeout_reported_and_healed(h):
         assert ht.get_async_status() == S.success
     finally:
         ht.close()
+        del ws
