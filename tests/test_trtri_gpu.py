@@ -250,14 +250,15 @@ def _bidiagonal(n, uplo, diag_value, poison):
 @pytest.mark.parametrize("diag", ["N", "U"])
 def test_split_products_exact_and_repeatable(h, uplo, diag):
     """Orders whose doubling products run on the DMA kernel with the diagonal
-    blocks of the triangular factors masked in the same kernel, at n = 4096
-    with the top level's K ranges split over several workgroups (even leading
-    dimensions), and on the masked kernels below (n = 1024, 2048).  The inverse of the bidiagonal
+    blocks of the triangular factors masked in the same kernel: with the K
+    steps of the top levels balanced over the workgroups (n = 2048, 4096; at
+    n = 3000 the uneven top group of the upper case, a transposed operand and
+    tiles cut by the edges), and on the masked kernels below (n = 1024).  The inverse of the bidiagonal
     I - J is the all-ones triangle: every partial sum is a small integer, so
     the result is exact whatever the split, and NaN in A's opposite triangle
     (and, for a unit diagonal, 3 on A's diagonal) must never enter.  A second
     call gives the same bits."""
-    for n in (1024, 2048, 4096):
+    for n in (1024, 2048, 3000, 4096):
         say("trtri split", uplo, diag, n)
         dA = _bidiagonal(n, uplo, 3.0 if diag == "U" else 1.0, float("nan"))
         dI = torch.full((n * n,), 5.0, dtype=torch.float64, device="cuda")
@@ -277,7 +278,7 @@ def test_split_products_exact_and_repeatable(h, uplo, diag):
 def test_split_products_accuracy(h, uplo):
     """Random well-conditioned triangles at n = 2048 and 4096 (split top level):
     ||invA T - I||_max on the device, and bitwise equal results on a repeat."""
-    for n in (2048, 4096):
+    for n in (2048, 3000, 4096):
         say("trtri split accuracy", uplo, n)
         dA = torch.empty(n * n, dtype=torch.float64, device="cuda")
         assert rb.synth_tri(h, uplo, "uniform", n, 7, dA.data_ptr(), n, scale=1.0 / n) \
