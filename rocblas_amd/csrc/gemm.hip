@@ -8,6 +8,10 @@
 // staged through LDS (double buffered), with the next K step's global loads in
 // flight during the current step's MFMAs.  Workgroup ids are remapped so that
 // each XCD's L2 sees contiguous runs of tiles, grouped by row panel.
+// A triangular operand (trtri's doubling products, the mixed form's
+// residuals) limits each tile's K range to the triangle and is masked in its
+// diagonal block; such products with few tiles split their tiles' K ranges or
+// balance their K steps over the workgroups.
 #include "device_common.hpp"
 
 #include <cstdlib>
