@@ -759,16 +759,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 }
 
 // The split tiles of a balanced product: C = alpha * (the pieces' sums, in
-// workgroup order) + beta * C.  One workgroup per tile and instance; tiles
-// that one workgroup computed whole return at once.
+// workgroup order) + beta * C.  Four workgroups per tile and instance, each
+// for a quarter of every thread's 64 values (more loads in flight than one
+// workgroup per tile); tiles that one workgroup computed whole return at once.
 template <int TRI>
 __global__ void __launch_bounds__(256)
     gemm_dma_reduce_kernel(const GemmArgsT<double, double, double> a)
 {
     constexpr bool      TRI_A = TRI == GT_A_LOWER || TRI == GT_A_UPPER;
+    constexpr int       NV    = 16; // values per thread per workgroup
     const BalancedSplit z     = BalancedSplit::of<TRI>(a);
-    const int           b     = blockIdx.x / z.total;
-    const int           t     = blockIdx.x - b * z.total;
+    const int           qtr   = blockIdx.x % 4;
+    const int           tb    = blockIdx.x / 4;
+    const int           b     = tb / z.total;
+    const int           t     = tb - b * z.total;
     const int           q     = t / z.other;
     const int           i     = t - q * z.other;
     int64_t             ts    = int64_t(b) * z.per_inst;
@@ -791,28 +795,29 @@ __global__ void __launch_bounds__(256)
     auto piece = [&](int64_t g) {
         return a.split_ws + (2 * g + (z.first(g) >= ts ? 0 : 1)) * int64_t(16384) + tid;
     };
-    double        sum[64];
-    const double* p0 = piece(w0);
+    double        sum[NV];
+    const double* p0 = piece(w0) + qtr * NV * 256;
 #pragma unroll
-    for(int v = 0; v < 64; ++v)
-        sum[v] = p0[v * 256];
+    for(int u = 0; u < NV; ++u)
+        sum[u] = p0[u * 256];
     for(int64_t g = w0 + 1; g <= wl; ++g)
     {
-        const double* pg = piece(g);
+        const double* pg = piece(g) + qtr * NV * 256;
 #pragma unroll
-        for(int v = 0; v < 64; ++v)
-            sum[v] += pg[v * 256];
+        for(int u = 0; u < NV; ++u)
+            sum[u] += pg[u * 256];
     }
 #pragma unroll
-    for(int v = 0; v < 64; ++v)
+    for(int u = 0; u < NV; ++u)
     {
+        const int v  = qtr * NV + u;
         const int i4 = v / 16, j4 = (v / 4) % 4, r = v % 4;
         const int row = m0 + (w % 2) * 64 + i4 * 16 + (lane & 15);
         const int col = n0 + (w / 2) * 64 + j4 * 16 + Mfma<double>::row(lane, r);
         if(row < a.M && col < a.N)
         {
             double& cref = Cp[row + int64_t(col) * a.ldc];
-            double  out  = a.alpha * sum[v];
+            double  out  = a.alpha * sum[u];
             if(a.beta != 0.0)
                 out = fma(a.beta, cref, out);
             cref = out;
@@ -1341,7 +1346,7 @@ static void launch_balanced_ops(rocblas_handle h, const GemmArgsT<double, double
     hipLaunchKernelGGL((gemm_dma_balanced_kernel<OPA, OPB, TRI>), dim3(unsigned(d.split_w)),
                        dim3(256), 0, h->stream, d);
     hipLaunchKernelGGL((gemm_dma_reduce_kernel<TRI>),
-                       dim3(unsigned(int64_t(d.tiles_m) * d.tiles_n * d.batch)), dim3(256), 0,
+                       dim3(unsigned(4 * int64_t(d.tiles_m) * d.tiles_n * d.batch)), dim3(256), 0,
                        h->stream, d);
 }
 
