@@ -1284,11 +1284,10 @@ static rocblas_status launch_gemm_dma_split(rocblas_handle h, const GemmArgsT<do
     return launch_status();
 }
 
-// Shortest K of a triangular product balanced over the workgroups; shorter
-// ones run on the masked register-staged kernels, which measured as fast or
-// faster there (K = 512, 4 groups of 4 x 4 tiles: 83 vs 92 us for the two
-// products of a level, the reduce passes included).  ROCBLAS_AMD_BALANCED_MIN_K
-// overrides it for A/B comparisons.
+// Shortest K of a triangular product balanced over the workgroups whatever
+// its tile count (half of it with 64 tiles or more); shorter ones run on the
+// masked register-staged kernels, which measured as fast or faster there.
+// ROCBLAS_AMD_BALANCED_MIN_K overrides it for A/B comparisons.
 static int balanced_min_k()
 {
     static const int v = [] {
@@ -1305,10 +1304,15 @@ static int64_t balanced_shape_workgroups(rocblas_handle h, int M, int N, int K, 
 {
     const int kind = tri & (GT_A_LOWER | GT_A_UPPER | GT_B_LOWER | GT_B_UPPER);
     if(!kind || !gemm_dma_mode() || !gemm_split_min() || M < 128 || N < 128
-       || K < balanced_min_k() || K % 8)
+       || K < balanced_min_k() / 2 || K % 8)
         return 0;
     const int64_t tm = (M + 127) / 128, tn = (N + 127) / 128;
     if(tm * tn * batch >= 2 * int64_t(h->num_cus) || tm * tn * batch > kSplitCounters)
+        return 0;
+    // between half the minimum K and the minimum, only with 64 tiles or more
+    // (n = 3000 lower, K = 952 over 128 tiles: 0.72 -> 0.62 ms; K = 512 over
+    // 16 or 32 tiles ran slower than the masked kernels)
+    if(K < balanced_min_k() && tm * tn * batch < 64)
         return 0;
     const bool onA   = kind == GT_A_LOWER || kind == GT_A_UPPER;
     const bool toEnd = kind == GT_A_LOWER || kind == GT_B_UPPER;
